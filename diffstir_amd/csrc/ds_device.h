@@ -1,0 +1,446 @@
+// ds_device.h -- device-side building blocks of the MLS-MPM stirring substep (gfx950).
+//
+// Shared by every kernel of libdiffstir_hip.so: quadratic B-spline weights, the three
+// rheology models and their partial derivatives, the FP64 3x3 signed SVD and its backward,
+// and the per-node grid update with its adjoint.  FP32 everywhere except the SVD, its
+// backward and the pressure term built from J (FP64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ds {
+
+enum Species : int { LIQUID = 0, STICK = 1, CONTAINER = 2 };
+
+// State layout: component-major SoA over all particles of all instances, 24 floats each.
+enum StateSlot : int { SX = 0, SV = 3, SC = 6, SF = 15, NSLOT = 24 };
+
+// One contiguous run of particles of one (instance, species); runs start on a multiple of
+// 64 so a wavefront never mixes instances or species.
+struct Group {
+    int inst;
+    int species;
+    int offset;
+    int count;
+};
+
+// Scene constants in the form the kernels use.
+struct DevScene {
+    int n_grid, bound, model, k_from_param;
+    float dx, inv_dx, dt;
+    float gdt;          // dt * gravity: added to the liquid's y velocity (simulation_functions.py:152)
+    float wall_r2, wall_cx, wall_cz;
+    float m_liq, m_stick, m_cont;
+    float alpha;        // -dt * vol_liq * 4 * inv_dx^2 (simulation_functions.py:150)
+    float c4;           // 4 * inv_dx (simulation_functions.py:244)
+    double k_bulk;
+};
+
+// ---------------------------------------------------------------------------- B-spline
+
+// Quadratic B-spline weights of one particle (simulation_functions.py:108-110): base is the
+// truncating cast of x/dx - 0.5.  Returns false when the 3x3x3 stencil leaves the grid or the
+// position is not finite; base is then clamped so every access stays in bounds.
+struct Stencil {
+    int b[3];
+    float fx[3];
+    float w[3][3];
+    float dw[3][3];   // d w / d fx
+};
+
+__device__ __forceinline__ bool stencil(const DevScene& sc, float x0, float x1, float x2, Stencil& s)
+{
+    const float xs[3] = {x0, x1, x2};
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float X = xs[a] * sc.inv_dx;
+        bool fin = isfinite(X);
+        int b = fin ? (int)(X - 0.5f) : 0;
+        if (!fin || b < 0 || b > sc.n_grid - 3) {
+            ok = false;
+            b = b < 0 ? 0 : (b > sc.n_grid - 3 ? sc.n_grid - 3 : b);
+            if (!fin) X = (float)b + 1.0f;
+        }
+        s.b[a] = b;
+        float f = X - (float)b;
+        s.fx[a] = f;
+        s.w[a][0] = 0.5f * (1.5f - f) * (1.5f - f);
+        s.w[a][1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+        s.w[a][2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+        s.dw[a][0] = f - 1.5f;
+        s.dw[a][1] = -2.0f * (f - 1.0f);
+        s.dw[a][2] = f - 0.5f;
+    }
+    return ok;
+}
+
+// ---------------------------------------------------------------------------- rheology
+
+// eta(gamma; params) with its partial derivatives (simulation_functions.py:119-146).
+struct Visc {
+    float eta;
+    float d_dg;
+    float d_dth[4];
+};
+
+__device__ __forceinline__ Visc viscosity(int model, const float th[4], float g)
+{
+    Visc r;
+    r.d_dth[0] = r.d_dth[1] = r.d_dth[2] = r.d_dth[3] = 0.0f;
+    if (model == 0) {           // Carreau
+        float e0 = th[0], ei = th[1], n = th[2], lam = th[3];
+        float lg = lam * g;
+        float q = 1.0f + lg * lg;
+        float e = (n - 1.0f) * 0.5f;
+        float lq = __logf(q);
+        float qe = __expf(e * lq);
+        float dq = e * qe / q;              // d q^e / dq
+        r.eta = ei + (e0 - ei) * qe;
+        r.d_dth[0] = qe;
+        r.d_dth[1] = 1.0f - qe;
+        r.d_dth[2] = (e0 - ei) * qe * lq * 0.5f;
+        r.d_dth[3] = (e0 - ei) * dq * 2.0f * lg * g;
+        r.d_dg = (e0 - ei) * dq * 2.0f * lg * lam;
+    } else if (model == 1) {    // Cross (exponent n - 1, simulation_functions.py:134)
+        float e0 = th[0], ei = th[1], n = th[2], lam = th[3];
+        float lg = lam * g;
+        float llg = __logf(lg);
+        float rr = __expf((n - 1.0f) * llg);
+        float D = 1.0f + rr;
+        float iD = 1.0f / D;
+        float c = -(e0 - ei) * iD * iD;
+        float dr = (n - 1.0f) * rr / lg;    // d rr / d lg
+        r.eta = ei + (e0 - ei) * iD;
+        r.d_dth[0] = iD;
+        r.d_dth[1] = 1.0f - iD;
+        r.d_dth[2] = c * rr * llg;
+        r.d_dth[3] = c * dr * g;
+        r.d_dg = c * dr * lam;
+    } else {                    // Herschel-Bulkley
+        float K = th[0], ty = th[1], n = th[2];
+        float lgm = __logf(g);
+        float gn = __expf((n - 1.0f) * lgm);
+        float de = K * gn;
+        float ig = 1.0f / g;
+        r.eta = ty * ig;
+        r.d_dth[1] = ig;
+        r.d_dg = -ty * ig * ig;
+        if (de > 0.0f) {
+            r.eta += de;
+            r.d_dth[0] = gn;
+            r.d_dth[2] = de * lgm;
+            r.d_dg += (n - 1.0f) * de * ig;
+        }
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------- FP64 SVD
+
+// One cyclic-Jacobi rotation on the symmetric 3x3 A (stored a00,a11,a22,a01,a02,a12),
+// annihilating A[p][q] and accumulating the rotation into the columns of V
+// (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.5.2).
+template <int P, int Q>
+__device__ __forceinline__ void jacobi_rot(double A[3][3], double V[3][3])
+{
+    constexpr int R = 3 - P - Q;
+    double apq = A[P][Q];
+    if (apq == 0.0) return;
+    double theta = (A[Q][Q] - A[P][P]) / (2.0 * apq);
+    double t;
+    if (fabs(theta) > 1e150) {
+        t = 0.5 / theta;
+    } else {
+        t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+        t = theta < 0.0 ? -t : t;
+    }
+    double c = 1.0 / sqrt(t * t + 1.0);
+    double s = t * c;
+    A[P][P] -= t * apq;
+    A[Q][Q] += t * apq;
+    A[P][Q] = A[Q][P] = 0.0;
+    double arp = A[R][P], arq = A[R][Q];
+    A[R][P] = A[P][R] = c * arp - s * arq;
+    A[R][Q] = A[Q][R] = s * arp + c * arq;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double vp = V[i][P], vq = V[i][Q];
+        V[i][P] = c * vp - s * vq;
+        V[i][Q] = s * vp + c * vq;
+    }
+}
+
+__device__ __forceinline__ void swap_cols(double V[3][3], double S[3], int a, int b)
+{
+    double t = S[a]; S[a] = S[b]; S[b] = t;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double u = V[i][a];
+        V[i][a] = V[i][b];
+        V[i][b] = -u;   // a swap with one sign flip keeps det V = +1
+    }
+}
+
+// Signed SVD in FP64: F = U diag(S) V^T with U, V rotations, S[0] >= S[1] >= |S[2]| and
+// S[2] < 0 when det F < 0.  Cyclic Jacobi on F^T F gives V and S^2; U comes from a
+// Gram-Schmidt QR of F V, the construction of McAdams et al., "Computing the Singular Value
+// Decomposition of 3x3 matrices with minimal branching and elementary floating point
+// operations" (2011), here with exact FP64 rotations.  S[0]*S[1]*S[2] = det F for any V
+// because F V = U R with R upper triangular and diag(R) = S.
+__device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3], double Vo[9])
+{
+    double A[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            A[i][j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
+    double V[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    for (int sweep = 0; sweep < 8; sweep++) {
+        double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-32 * dia) break;
+        jacobi_rot<0, 1>(A, V);
+        jacobi_rot<0, 2>(A, V);
+        jacobi_rot<1, 2>(A, V);
+    }
+    double L[3] = {A[0][0], A[1][1], A[2][2]};
+    if (L[0] < L[1]) swap_cols(V, L, 0, 1);
+    if (L[0] < L[2]) swap_cols(V, L, 0, 2);
+    if (L[1] < L[2]) swap_cols(V, L, 1, 2);
+    // B = F V, columns b0, b1, b2
+    double b[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            b[c][i] = F[i * 3 + 0] * V[0][c] + F[i * 3 + 1] * V[1][c] + F[i * 3 + 2] * V[2][c];
+    double u0[3], u1[3], u2[3];
+    double n0 = sqrt(b[0][0] * b[0][0] + b[0][1] * b[0][1] + b[0][2] * b[0][2]);
+    if (n0 > 1e-300) {
+        double in = 1.0 / n0;
+        u0[0] = b[0][0] * in; u0[1] = b[0][1] * in; u0[2] = b[0][2] * in;
+    } else {
+        u0[0] = 1.0; u0[1] = 0.0; u0[2] = 0.0;
+    }
+    double d01 = u0[0] * b[1][0] + u0[1] * b[1][1] + u0[2] * b[1][2];
+    double r1[3] = {b[1][0] - d01 * u0[0], b[1][1] - d01 * u0[1], b[1][2] - d01 * u0[2]};
+    double n1 = sqrt(r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+    if (n1 > 1e-300) {
+        double in = 1.0 / n1;
+        u1[0] = r1[0] * in; u1[1] = r1[1] * in; u1[2] = r1[2] * in;
+    } else {
+        // any unit vector orthogonal to u0
+        double ax = fabs(u0[0]) < 0.9 ? 1.0 : 0.0, ay = 1.0 - ax;
+        double t[3] = {u0[1] * 0.0 - u0[2] * ay, u0[2] * ax - u0[0] * 0.0, u0[0] * ay - u0[1] * ax};
+        double nt = 1.0 / sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        u1[0] = t[0] * nt; u1[1] = t[1] * nt; u1[2] = t[2] * nt;
+    }
+    u2[0] = u0[1] * u1[2] - u0[2] * u1[1];
+    u2[1] = u0[2] * u1[0] - u0[0] * u1[2];
+    u2[2] = u0[0] * u1[1] - u0[1] * u1[0];
+    S[0] = n0;
+    S[1] = n1;
+    S[2] = u2[0] * b[2][0] + u2[1] * b[2][1] + u2[2] * b[2][2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        U[i * 3 + 0] = u0[i];
+        U[i * 3 + 1] = u1[i];
+        U[i * 3 + 2] = u2[i];
+        Vo[i * 3 + 0] = V[i][0];
+        Vo[i * 3 + 1] = V[i][1];
+        Vo[i * 3 + 2] = V[i][2];
+    }
+}
+
+// clamp of simulation_functions.py:61-67
+__device__ __forceinline__ double svd_clamp(double a)
+{
+    return a >= 0.0 ? fmax(a, 1e-6) : fmin(a, -1e-6);
+}
+
+// SVD backward of simulation_functions.py:70-87 (PyTorch's formula):
+//   gF = U (f o (U^T gU - gU^T U)) S V^T + U S (f o (V^T gV - gV^T V)) V^T + U diag(gS) V^T
+// with f_ij = 1 / clamp(s_j^2 - s_i^2), f_ii = 0.
+__device__ __forceinline__ void svd3_backward(const double U[9], const double S[3], const double V[9],
+                                              const double gU[9], const double gS[3], const double gV[9],
+                                              double gF[9])
+{
+    double s2[3] = {S[0] * S[0], S[1] * S[1], S[2] * S[2]};
+    double f[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) f[i][j] = (i == j) ? 0.0 : 1.0 / svd_clamp(s2[j] - s2[i]);
+    double Ku[3][3], Kv[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double a = 0.0, bb = 0.0, c = 0.0, d = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                a += U[k * 3 + i] * gU[k * 3 + j];     // (U^T gU)_ij
+                bb += gU[k * 3 + i] * U[k * 3 + j];    // (gU^T U)_ij
+                c += V[k * 3 + i] * gV[k * 3 + j];     // (V^T gV)_ij
+                d += gV[k * 3 + i] * V[k * 3 + j];     // (gV^T V)_ij
+            }
+            Ku[i][j] = f[i][j] * (a - bb);
+            Kv[i][j] = f[i][j] * (c - d);
+        }
+    // inner = Ku S + S Kv + diag(gS)
+    double M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) M[i][j] = Ku[i][j] * S[j] + S[i] * Kv[i][j] + (i == j ? gS[i] : 0.0);
+    // gF = U M V^T
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double um = U[i * 3 + 0] * M[0][k] + U[i * 3 + 1] * M[1][k] + U[i * 3 + 2] * M[2][k];
+                acc += um * V[j * 3 + k];
+            }
+            gF[i * 3 + j] = acc;
+        }
+}
+
+// The same backward when only the singular values carry an adjoint (gU = gV = 0): the
+// u- and v-terms of svd3_backward vanish identically, leaving U diag(gS) V^T.
+__device__ __forceinline__ void svd3_backward_sigma(const double U[9], const double V[9], const double gS[3],
+                                                    double gF[9])
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            gF[i * 3 + j] = U[i * 3 + 0] * gS[0] * V[j * 3 + 0] + U[i * 3 + 1] * gS[1] * V[j * 3 + 1]
+                          + U[i * 3 + 2] * gS[2] * V[j * 3 + 2];
+}
+
+// ---------------------------------------------------------------------------- liquid stress
+
+// Everything P2G needs from one liquid particle (simulation_functions.py:111-153):
+// F_tmp = (I + dt C) F, J from the FP64 SVD of F_tmp, gamma = |C|_F + 1e-6, eta(gamma),
+// S = eta sym(C) + I (k/2)(J - 1/J - 1) J, affine A = alpha S + m C.
+struct LiquidP2G {
+    float Fn[9];
+    float A[9];
+    float eta;
+    float gamma;
+    double J;
+    double U[9], S[3], V[9];
+};
+
+__device__ __forceinline__ void liquid_p2g(const DevScene& sc, const float th[4], double kb, const float C[9],
+                                           const float F[9], LiquidP2G& q)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float s = F[i * 3 + j];
+#pragma unroll
+            for (int k = 0; k < 3; k++) s = fmaf(sc.dt * C[i * 3 + k], F[k * 3 + j], s);
+            q.Fn[i * 3 + j] = s;
+        }
+    double Fd[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Fd[i] = (double)q.Fn[i];
+    svd3(Fd, q.U, q.S, q.V);
+    double J = q.S[0] * q.S[1] * q.S[2];
+    q.J = J;
+    float n2 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; i++) n2 = fmaf(C[i], C[i], n2);
+    q.gamma = sqrtf(n2) + 1e-6f;
+    Visc vv = viscosity(sc.model, th, q.gamma);
+    q.eta = vv.eta;
+    float pr = (float)((kb * 0.5) * (J - 1.0 / J - 1.0) * J);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float s = q.eta * 0.5f * (C[i * 3 + j] + C[j * 3 + i]) + (i == j ? pr : 0.0f);
+            q.A[i * 3 + j] = sc.alpha * s + sc.m_liq * C[i * 3 + j];
+        }
+}
+
+// ---------------------------------------------------------------------------- grid update
+
+// Per-node grid update (simulation_functions.py:199-225) with the branch record the adjoint
+// replays: v = p / (m + 1e-10); outside the cylinder radius the normal component is removed;
+// box bounds zero outward components; the top face zeroes all three.
+struct NodeRec {
+    float inv_m;
+    float v0[3];
+    float n[3];
+    bool wall;
+    bool mask[3];
+};
+
+__device__ __forceinline__ void gridop_node(const DevScene& sc, float4 P, int i, int j, int k, float vo[3], NodeRec& r)
+{
+    r.inv_m = 1.0f / (P.w + 1e-10f);
+    float v[3] = {r.inv_m * P.x, r.inv_m * P.y, r.inv_m * P.z};
+    r.v0[0] = v[0]; r.v0[1] = v[1]; r.v0[2] = v[2];
+    float d0 = (float)i * sc.dx - sc.wall_cx;
+    float d2 = (float)k * sc.dx - sc.wall_cz;
+    float dd = d0 * d0 + d2 * d2;
+    r.wall = dd > sc.wall_r2;
+    r.n[0] = r.n[1] = r.n[2] = 0.0f;
+    if (r.wall) {
+        float in = 1.0f / sqrtf(dd);
+        r.n[0] = d0 * in;
+        r.n[2] = d2 * in;
+        float vn = v[0] * r.n[0] + v[2] * r.n[2];
+        v[0] -= r.n[0] * vn;
+        v[2] -= r.n[2] * vn;
+    }
+    int n = sc.n_grid, bd = sc.bound;
+    r.mask[0] = r.mask[1] = r.mask[2] = false;
+    if (i < bd && v[0] < 0.0f) { v[0] = 0.0f; r.mask[0] = true; }
+    if (i > n - bd && v[0] > 0.0f) { v[0] = 0.0f; r.mask[0] = true; }
+    if (k < bd && v[2] < 0.0f) { v[2] = 0.0f; r.mask[2] = true; }
+    if (k > n - bd && v[2] > 0.0f) { v[2] = 0.0f; r.mask[2] = true; }
+    if (j < bd && v[1] < 0.0f) { v[1] = 0.0f; r.mask[1] = true; }
+    if (j > n - bd && v[1] > 0.0f) {
+        v[0] = v[1] = v[2] = 0.0f;
+        r.mask[0] = r.mask[1] = r.mask[2] = true;
+    }
+    vo[0] = v[0]; vo[1] = v[1]; vo[2] = v[2];
+}
+
+// Adjoint of gridop_node: given dL/dv_out, returns dL/d(momentum) in xyz and dL/d(mass) in w.
+__device__ __forceinline__ float4 gridop_node_adj(const NodeRec& r, float g0, float g1, float g2)
+{
+    float g[3] = {r.mask[0] ? 0.0f : g0, r.mask[1] ? 0.0f : g1, r.mask[2] ? 0.0f : g2};
+    if (r.wall) {
+        float gn = g[0] * r.n[0] + g[2] * r.n[2];
+        g[0] -= r.n[0] * gn;
+        g[2] -= r.n[2] * gn;
+    }
+    float4 o;
+    o.x = r.inv_m * g[0];
+    o.y = r.inv_m * g[1];
+    o.z = r.inv_m * g[2];
+    o.w = -r.inv_m * (g[0] * r.v0[0] + g[1] * r.v0[1] + g[2] * r.v0[2]);
+    return o;
+}
+
+// ---------------------------------------------------------------------------- wave reductions
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace ds

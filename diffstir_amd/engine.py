@@ -1,0 +1,182 @@
+"""`Engine`: a batch of stirring instances on one GPU, over the C ABI of libdiffstir_hip.so.
+
+One Engine owns one device context.  All instances share the grid, time step, rheology
+model and species masses of their SceneConfig; each has its own particles, stick trajectory
+and constitutive parameters (the reference runs exactly one instance,
+src/simulator/initialization.py:123-165).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .scene import Scene, SceneConfig
+
+
+def scene_desc(cfg: SceneConfig) -> _lib.SceneDesc:
+    return _lib.SceneDesc(int(cfg.n_grid), int(cfg.bound), int(cfg.model), int(bool(cfg.k_from_param)),
+                          float(cfg.dt), float(cfg.gravity), float(cfg.k_bulk), float(cfg.wall_r),
+                          float(cfg.wall_cx), float(cfg.wall_cz), float(cfg.m_liq), float(cfg.vol_liq),
+                          float(cfg.m_stick), float(cfg.m_cont))
+
+
+def _f32(a, shape):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(shape))
+
+
+def _p32(a):
+    return a.ctypes.data_as(_lib.F32P) if a is not None and a.size else None
+
+
+def _p64(a):
+    return a.ctypes.data_as(_lib.F64P)
+
+
+_SHARED = ("n_grid", "dt", "bound", "gravity", "model", "k_bulk", "k_from_param", "wall_r", "wall_cx", "wall_cz",
+           "m_liq", "vol_liq", "m_stick", "m_cont")
+
+
+class Engine:
+    """B instances advanced together; forward/backward run every instance."""
+
+    def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None):
+        self.lib = _lib.load()
+        scenes = list(scenes)
+        if not scenes:
+            raise ValueError("need at least one scene")
+        self.scenes = scenes
+        cfg = scenes[0].cfg
+        for s in scenes[1:]:
+            for k in _SHARED:
+                if getattr(s.cfg, k) != getattr(cfg, k):
+                    raise ValueError(f"instances must share SceneConfig.{k}")
+        self.cfg = cfg
+        self.B = len(scenes)
+        self.T = int(max_steps if max_steps is not None else min(s.max_steps for s in scenes))
+        self.K = int(ckpt_every)
+        nl = (C.c_int32 * self.B)(*[s.n_liquid for s in scenes])
+        ns = (C.c_int32 * self.B)(*[s.n_stick for s in scenes])
+        nc = (C.c_int32 * self.B)(*[s.n_container for s in scenes])
+        self._desc = scene_desc(cfg)
+        h = C.c_void_p()
+        _lib.check(self.lib.ds_create(C.byref(self._desc), self.B, nl, ns, nc, self.T, self.K, C.byref(h)), None)
+        self.ctx = h
+        for b, s in enumerate(scenes):
+            xl = _f32(s.x_liquid, (-1, 3))
+            xs = _f32(s.x_stick, (-1, 3))
+            xc = _f32(s.x_container, (-1, 3))
+            self._check(self.lib.ds_set_particles(self.ctx, b, _p32(xl), _p32(xs), _p32(xc)))
+            tr = _f32(s.traj[:self.T], (self.T, 3))
+            self._check(self.lib.ds_set_trajectory(self.ctx, b, _p32(tr)))
+        self.set_params(np.stack([np.asarray(s.params, dtype=np.float64) for s in scenes]))
+
+    # ------------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.ds_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, code):
+        _lib.check(code, self.ctx)
+
+    # ------------------------------------------------------------------ API
+    @property
+    def n_liquid(self) -> List[int]:
+        return [s.n_liquid for s in self.scenes]
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(self.B, 4))
+        self._check(self.lib.ds_set_params(self.ctx, _p64(p)))
+        self.params = p.copy()
+
+    def reset(self):
+        """initialize_ti_field (simulation_functions.py:301-319)."""
+        self._check(self.lib.ds_reset(self.ctx))
+
+    def forward(self, nsteps: Optional[int] = None):
+        """Advance nsteps substeps (default: to the end of the trajectory)."""
+        if nsteps is None:
+            nsteps = self.T - 1 - self.step
+        self._check(self.lib.ds_forward(self.ctx, int(nsteps)))
+
+    @property
+    def step(self) -> int:
+        s = C.c_int32()
+        self._check(self.lib.ds_current_step(self.ctx, C.byref(s)))
+        return int(s.value)
+
+    def forces(self) -> np.ndarray:
+        """[B, T, 3] stick forces; row s is m_stick/dt * sum_p dv[s, p]."""
+        out = np.zeros((self.B, self.T, 3))
+        self._check(self.lib.ds_get_forces(self.ctx, _p64(out)))
+        return out
+
+    def backward(self, dl_dforces) -> np.ndarray:
+        """dL/dparams [B, 4] for the recorded rollout, given dL/dforces [B, T, 3]."""
+        g = np.ascontiguousarray(np.asarray(dl_dforces, dtype=np.float64).reshape(self.B, self.T, 3))
+        out = np.zeros((self.B, 4))
+        self._check(self.lib.ds_backward(self.ctx, _p64(g), _p64(out)))
+        return out
+
+    def timing(self):
+        f = C.c_double()
+        b = C.c_double()
+        self._check(self.lib.ds_last_timing(self.ctx, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    def read_liquid(self, inst: int = 0) -> dict:
+        n = self.scenes[inst].n_liquid
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        Cm = np.zeros((n, 9), np.float32)
+        F = np.zeros((n, 9), np.float32)
+        self._check(self.lib.ds_read_liquid(self.ctx, inst, _p32(x), _p32(v), _p32(Cm), _p32(F)))
+        return {"x": x, "v": v, "C": Cm, "F": F}
+
+    def read_stick(self, inst: int = 0) -> dict:
+        n = self.scenes[inst].n_stick
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        Cm = np.zeros((n, 9), np.float32)
+        self._check(self.lib.ds_read_stick(self.ctx, inst, _p32(x), _p32(v), _p32(Cm)))
+        return {"x": x, "v": v, "C": Cm}
+
+    def synchronize(self):
+        self._check(self.lib.ds_synchronize(self.ctx))
+
+
+def svd3(F: np.ndarray):
+    """FP64 signed SVD of [n, 3, 3] matrices on the device: returns U, S, V."""
+    lib = _lib.load()
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(-1, 9))
+    n = F.shape[0]
+    U = np.zeros((n, 9))
+    S = np.zeros((n, 3))
+    V = np.zeros((n, 9))
+    _lib.check(lib.ds_svd3(_p64(F), n, _p64(U), _p64(S), _p64(V)), None)
+    return U.reshape(n, 3, 3), S, V.reshape(n, 3, 3)
+
+
+def svd3_backward(U, S, V, gU, gS, gV):
+    """FP64 SVD backward on the device (simulation_functions.py:70-87)."""
+    lib = _lib.load()
+    arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in (U, S, V, gU, gS, gV)]
+    n = arrs[1].reshape(-1, 3).shape[0]
+    gF = np.zeros((n, 9))
+    _lib.check(lib.ds_svd3_backward(*[_p64(a) for a in arrs[:6]], n, _p64(gF)), None)
+    return gF.reshape(n, 3, 3)

@@ -1,0 +1,118 @@
+/*
+ * diffstir.h -- C ABI of the MI355X stirring simulator (libdiffstir_hip.so).
+ *
+ * The reference exposes one operator pair for this path: Taichi's custom-gradient
+ * forward(s)/backward(s) (src/simulator/simulation_functions.py:341-365) over module-global
+ * fields (src/simulator/initialization.py:123-165), recorded by ti.ad.Tape
+ * (src/gradient_descent_module.py:103-107).  This library replaces that whole layer: a context
+ * owns the device state of B independent instances (the reference has B = 1), advances them a
+ * number of substeps at a time, records the stick force, and runs the checkpointed adjoint.
+ *
+ * Conventions
+ *  - Every call returns 0 on success or a DS_E* code; ds_last_error() gives the message.
+ *  - Host pointers only (plain C arrays); the context owns all device memory.
+ *  - Calls on one context are serialised on the context's HIP stream; a context is used by
+ *    one host thread at a time.
+ *  - Forces: forces[b][s][3] for s = 0..max_steps-1, F(s) = m_stick/dt * sum_p dv[s,p]
+ *    (simulation_functions.py:330-336, force_calculation_and_animation_module.py:16-19);
+ *    row 0 is 0 because the stick starts with dv = 0.
+ *  - Matrices are row-major 3x3; particle arrays are [n][3] / [n][9] float32.
+ */
+#ifndef DIFFSTIR_H
+#define DIFFSTIR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ds_ctx ds_ctx;
+
+enum {
+    DS_OK = 0,
+    DS_EINVAL = 1,     /* bad argument */
+    DS_EHIP = 2,       /* HIP runtime error */
+    DS_ENOMEM = 3,     /* device allocation failed */
+    DS_ESTATE = 4,     /* call out of order (e.g. backward before forward) */
+    DS_ENONFINITE = 5, /* a particle left the grid or became non-finite */
+};
+
+enum { DS_MODEL_CARREAU = 0, DS_MODEL_CROSS = 1, DS_MODEL_HERSCHEL_BULKLEY = 2 };
+
+/* Scene constants shared by all instances of a context.
+ * Reference: initialization.py:14-54 (constants), simulation_functions.py:104-225 (hard-coded
+ * k = 20000, wall radius 0.0415 - dx/2, bound 3, g = -9.81). */
+typedef struct ds_scene_desc {
+    int32_t n_grid;        /* nodes per axis; dx = 1/n_grid */
+    int32_t bound;         /* box-boundary width in nodes (initialization.py:15) */
+    int32_t model;         /* DS_MODEL_* (simulation_functions.py:120,129,138) */
+    int32_t k_from_param;  /* 1: bulk modulus = params[3] (Herschel-Bulkley only) */
+    double dt;
+    double gravity;        /* y acceleration added to liquid velocity (simulation_functions.py:152) */
+    double k_bulk;         /* bulk modulus when k_from_param == 0 (reference: 20000) */
+    double wall_r, wall_cx, wall_cz;   /* cylindrical slip wall (simulation_functions.py:205-210) */
+    double m_liq, vol_liq, m_stick, m_cont;   /* per-particle mass / volume per species */
+} ds_scene_desc;
+
+/* Create a context for n_inst instances with the given particle counts per species, a
+ * trajectory of max_steps frames (max_steps - 1 substeps, gradient_descent_module.py:105) and
+ * an adjoint checkpoint every ckpt_every substeps.  Replaces the field allocation of
+ * initialization.py:123-165 (which stores every step; here only checkpoints are kept). */
+int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid,
+              const int32_t* n_stick, const int32_t* n_container, int32_t max_steps,
+              int32_t ckpt_every, ds_ctx** out);
+void ds_destroy(ds_ctx* ctx);
+const char* ds_last_error(const ds_ctx* ctx);
+
+/* Initial particle positions of one instance ([n][3] each; NULL allowed for an empty species).
+ * Replaces initialize_objects' from_numpy uploads (simulation_functions.py:293-297). */
+int ds_set_particles(ds_ctx* ctx, int32_t inst, const float* x_liquid, const float* x_stick,
+                     const float* x_container);
+/* Stick trajectory of one instance, [max_steps][3] (operator_kinematic_seq, initialization.py:151). */
+int ds_set_trajectory(ds_ctx* ctx, int32_t inst, const float* traj);
+/* Constitutive parameters, [n_inst][4] (constitutive_parameters, initialization.py:134). */
+int ds_set_params(ds_ctx* ctx, const double* params);
+
+/* Step-0 state from the initial positions (v = 0, F = I, C = 0), forces zeroed, step counter
+ * reset.  Replaces initialize_ti_field (simulation_functions.py:301-319). */
+int ds_reset(ds_ctx* ctx);
+/* Advance every instance nsteps substeps from the current step, recording forces and storing
+ * a checkpoint every ckpt_every steps.  Replaces the loop of forward(s) calls
+ * (simulation_functions.py:341-348, gradient_descent_module.py:105-106). */
+int ds_forward(ds_ctx* ctx, int32_t nsteps);
+/* Number of substeps taken since ds_reset. */
+int ds_current_step(const ds_ctx* ctx, int32_t* step);
+/* Stick forces recorded so far, [n_inst][max_steps][3] (total_operator_dv * m / dt,
+ * force_calculation_and_animation_module.py:16-19). */
+int ds_get_forces(ds_ctx* ctx, double* forces);
+/* Adjoint of the whole recorded rollout: given dL/dforces [n_inst][max_steps][3], writes
+ * dL/dparams [n_inst][4].  Recomputes each checkpoint segment.  Replaces backward(s)
+ * (simulation_functions.py:351-365) as replayed by ti.ad.Tape (gradient_descent_module.py:103-107). */
+int ds_backward(ds_ctx* ctx, const double* dl_dforces, double* dl_dparams);
+/* Liquid state of one instance at the current step, in the order the particles were given.
+ * Any output pointer may be NULL.  x/v: [n][3], C/F: [n][9]. */
+int ds_read_liquid(ds_ctx* ctx, int32_t inst, float* x, float* v, float* C, float* F);
+/* Stick state of one instance at the current step ([n][3], [n][3], [n][9]). */
+int ds_read_stick(ds_ctx* ctx, int32_t inst, float* x, float* v, float* C);
+/* Wait for all work queued on the context's stream. */
+int ds_synchronize(ds_ctx* ctx);
+
+/* Timing of the most recent ds_forward / ds_backward, measured with HIP events on the
+ * context's stream: elapsed milliseconds and kernel launches. */
+int ds_last_timing(const ds_ctx* ctx, double* fwd_ms, double* bwd_ms);
+
+/* FP64 3x3 signed SVD on the device, for tests: F[n][9] -> U[n][9], S[n][3], V[n][9]
+ * with F = U diag(S) V^T, det U = det V = +1 and S[2] carrying the sign of det F.
+ * (The reference's svd(), simulation_functions.py:55-58, uses ti.svd in f32.) */
+int ds_svd3(const double* F, int32_t n, double* U, double* S, double* V);
+/* FP64 SVD backward on the device, for tests: given U, S, V and the adjoints gU, gS (diagonal,
+ * [n][3]), gV, writes gF.  simulation_functions.py:61-87 (clamped 1/(s_j^2 - s_i^2)). */
+int ds_svd3_backward(const double* U, const double* S, const double* V, const double* gU,
+                     const double* gS, const double* gV, int32_t n, double* gF);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFSTIR_H */

@@ -1,0 +1,176 @@
+"""GPU parity of the HIP substep and its adjoint against the FP64 CPU oracle.
+
+Tolerances (FP32 bulk arithmetic on the GPU against the FP64 oracle):
+  forces     max_s |F - F_oracle| <= 1e-4 * max_s |F_oracle|  (rows s >= 2; row 1 is the
+             stick's start-up spike, which would make any relative figure look small)
+  gradients  max_i |g_i - g_oracle_i| <= 1e-3 * max_i |g_oracle_i|
+These are the force-profile and parameter-gradient tolerances the design targets.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from diffstir_amd import scene as S
+from diffstir_amd.engine import Engine, svd3, svd3_backward
+from diffstir_amd.loss import compute_loss
+from oracle.oracle import Oracle, loss_and_grad
+from tests.helpers import force_err, gold_scene, grad_err
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def o64():
+    return Oracle("f64")
+
+
+def gpu_rollout_grad(sc, nsteps, target, ckpt=10):
+    with Engine([sc], ckpt_every=ckpt, max_steps=nsteps + 1) as e:
+        e.reset()
+        e.forward(nsteps)
+        F = e.forces()[0]
+        L, g = compute_loss(F, target, sc.cfg.loss_kind)
+        dth = e.backward(g[None])[0]
+    return F, L, dth
+
+
+@pytest.mark.parametrize("name", ["tiny_hb", "ref_carreau", "ref_cross", "ref_hb", "plain_n32_20"])
+def test_golden_forces_and_gradient(name):
+    sc, z = gold_scene(name)
+    n = int(z["nsteps"])
+    print(f"[gpu] golden {name}: {sc.n_liquid} liquid, {sc.n_stick} stick, {sc.n_container} container, {n} steps",
+          flush=True)
+    F, L, g = gpu_rollout_grad(sc, n, z["target"], ckpt=7)
+    assert F[0].tolist() == [0.0, 0.0, 0.0]
+    fe = force_err(F, z["forces"])
+    ge = grad_err(g, z["dl_dparams"])
+    print(f"  force err {fe:.3e}  loss {L:.8g} vs {float(z['loss']):.8g}  grad err {ge:.3e}")
+    assert fe <= FORCE_TOL
+    assert abs(L - float(z["loss"])) <= 1e-4 * abs(float(z["loss"]))
+    assert ge <= GRAD_TOL
+
+
+@pytest.mark.parametrize("seed,model", [(0, 2), (1, 0), (2, 1)])
+def test_live_oracle_random_scenes(o64, seed, model):
+    """Seeded variations checked against the oracle run live (not only the fixtures)."""
+    rng = np.random.default_rng(100 + seed)
+    if model == 2:
+        params = S.hb_params_draw(rng)
+        sc = S.tiny_scene(16, 30, seed=seed, params=params)
+    else:
+        sc = S.tiny_scene(16, 30, seed=seed)
+        cfg = dataclasses.replace(sc.cfg, model=model, k_from_param=False, k_bulk=1e3)
+        params = np.array([rng.uniform(0.5, 3.0), rng.uniform(0.0, 0.4), rng.uniform(0.5, 1.5),
+                           rng.uniform(0.5, 5.0)])
+        sc = dataclasses.replace(sc, cfg=cfg, params=params)
+    print(f"[gpu] live oracle seed {seed} model {model} params {params}", flush=True)
+    Fo = o64.rollout(sc, nsteps=30)
+    tgt = 0.7 * Fo
+    L0, g0 = loss_and_grad(Fo, tgt, sc.cfg.loss_kind)
+    _, dth_o = o64.rollout_grad(sc, g0, nsteps=30, ckpt=6)
+    F, L, dth = gpu_rollout_grad(sc, 30, tgt, ckpt=6)
+    print(f"  force err {force_err(F, Fo):.3e} grad err {grad_err(dth, dth_o):.3e}")
+    assert force_err(F, Fo) <= FORCE_TOL
+    assert grad_err(dth, dth_o) <= GRAD_TOL
+
+
+def test_batch_matches_single_instances():
+    """Uneven instances advanced together give each instance's own result."""
+    scenes = [S.tiny_scene(16, 25, seed=s, fill_height=h, params=S.hb_params_draw(np.random.default_rng(s)))
+              for s, h in [(0, 0.15), (1, 0.25), (2, 0.2)]]
+    print("[gpu] batch of 3 vs singles", [s.n_liquid for s in scenes], flush=True)
+    with Engine(scenes, ckpt_every=5) as e:
+        e.reset()
+        e.forward()
+        Fb = e.forces()
+        tg = [0.5 * Fb[b] for b in range(3)]
+        gs = np.stack([compute_loss(Fb[b], tg[b], 1)[1] for b in range(3)])
+        db = e.backward(gs)
+    for b, sc in enumerate(scenes):
+        F, L, d = gpu_rollout_grad(sc, 25, tg[b], ckpt=5)
+        assert force_err(Fb[b], F) <= 1e-5
+        assert grad_err(db[b], d) <= 1e-5
+
+
+def test_checkpoint_interval_does_not_change_gradient():
+    sc = S.tiny_scene(16, 24, seed=5)
+    print("[gpu] checkpoint interval invariance", flush=True)
+    F0, _, _ = gpu_rollout_grad(sc, 24, np.zeros((25, 3)), ckpt=24)
+    tgt = 0.5 * F0
+    grads = [gpu_rollout_grad(sc, 24, tgt, ckpt=k)[2] for k in (1, 5, 24)]
+    for g in grads[1:]:
+        assert grad_err(g, grads[0]) <= 1e-5
+
+
+def test_stick_alone_gives_centripetal_force():
+    """A stick with no liquid or container moves rigidly: its grid velocity is its own velocity,
+    so F(s) = M (v_kin(s-1) - v_kin(s-2)) / dt -> M w^2 R towards the circle's centre, and
+    F(1) = M v_kin(0) / dt (the start-up step, v = 0 at step 0)."""
+    sc = S.stirring_scene(32, 60, fill_radius=0.2, fill_height=0.2, stick_radius=0.05, path_rx=0.15,
+                          path_rz=0.15, omega=2 * np.pi * 5)
+    cfg = dataclasses.replace(sc.cfg, wall_r=10.0)
+    sc = dataclasses.replace(sc, cfg=cfg, x_liquid=np.zeros((0, 3), np.float32))
+    print(f"[gpu] stick alone: {sc.n_stick} particles", flush=True)
+    with Engine([sc], ckpt_every=10) as e:
+        e.reset()
+        e.forward()
+        F = e.forces()[0]
+    M = sc.cfg.m_stick * sc.n_stick
+    dt = sc.cfg.dt
+    tr = sc.traj.astype(np.float64)
+    vk = (tr[1:] - tr[:-1]) / np.float32(dt)
+    np.testing.assert_allclose(F[1], M * vk[0] / dt, rtol=1e-4)
+    acc = (vk[1:] - vk[:-1]) / dt      # F(s) for s >= 2 equals M * acc[s-2]
+    np.testing.assert_allclose(F[2:], M * acc[:F.shape[0] - 2], rtol=2e-2, atol=2e-3 * np.abs(M * acc).max())
+    w = 2 * np.pi * 5
+    R = 0.15
+    mag = np.linalg.norm(F[5:, [0, 2]], axis=1)
+    np.testing.assert_allclose(mag, M * w * w * R, rtol=2e-2)
+
+
+# ------------------------------------------------------------------------- FP64 SVD
+
+def _random_F(rng, n):
+    F = np.eye(3) + 0.3 * rng.normal(size=(n, 3, 3))
+    F[: n // 8] = np.eye(3)                                     # exactly degenerate
+    F[n // 8: n // 4] = np.eye(3) + 1e-7 * rng.normal(size=(n // 4 - n // 8, 3, 3))
+    F[n // 4: n // 4 + 8, :, 0] *= -1.0                          # inverted
+    F[n // 4 + 8: n // 4 + 16] *= 1e-3                           # tiny
+    return F
+
+
+def test_svd3_matches_lapack():
+    rng = np.random.default_rng(0)
+    F = _random_F(rng, 512)
+    U, Sg, V = svd3(F)
+    rec = U @ (Sg[:, :, None] * np.swapaxes(V, 1, 2))
+    assert np.abs(rec - F).max() <= 1e-12 * max(1.0, np.abs(F).max())
+    eye = np.eye(3)
+    assert np.abs(np.swapaxes(U, 1, 2) @ U - eye).max() < 1e-12
+    assert np.abs(np.swapaxes(V, 1, 2) @ V - eye).max() < 1e-12
+    np.testing.assert_allclose(np.linalg.det(U), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.linalg.det(V), 1.0, atol=1e-12)
+    ref = np.linalg.svd(F, compute_uv=False)
+    np.testing.assert_allclose(np.abs(Sg), ref, atol=1e-13, rtol=1e-12)
+    np.testing.assert_allclose(Sg.prod(1), np.linalg.det(F), rtol=1e-11, atol=1e-15)
+
+
+def test_svd3_backward_matches_torch_autograd():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(1)
+    n = 64
+    F = np.eye(3) + 0.5 * rng.normal(size=(n, 3, 3))
+    Ft = torch.tensor(F, dtype=torch.float64, requires_grad=True)
+    U, Sg, Vh = torch.linalg.svd(Ft)
+    V = Vh.transpose(1, 2)
+    gU = rng.normal(size=(n, 3, 3))
+    gS = rng.normal(size=(n, 3))
+    gV = rng.normal(size=(n, 3, 3))
+    L = (U * torch.tensor(gU)).sum() + (Sg * torch.tensor(gS)).sum() + (V * torch.tensor(gV)).sum()
+    (gF_t,) = torch.autograd.grad(L, Ft)
+    gF = svd3_backward(U.detach().numpy(), Sg.detach().numpy(), V.detach().numpy(), gU, gS, gV)
+    np.testing.assert_allclose(gF, gF_t.numpy(), rtol=1e-8, atol=1e-10)
