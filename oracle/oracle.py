@@ -69,6 +69,8 @@ class Oracle:
         self._p2g = getattr(self.lib, pre + "p2g")
         self._gather = getattr(self.lib, pre + "gather")
         self._eta.restype = C.c_double
+        self._eta.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)]
 
     # ------------------------------------------------------------------ marshalling
     @staticmethod
