@@ -5,7 +5,7 @@
 // every P2G contribution an atomic add in L2.  The tiled fused kernels (kernels_tiled.hip)
 // are checked against them; they also serve the recompute of the first and last step of
 // each adjoint segment.
-#include "ds_device.h"
+#include "ds_particle.h"
 
 namespace ds {
 
@@ -50,6 +50,23 @@ __global__ void __launch_bounds__(256) k_init_state(float* __restrict__ st, cons
 
 // ---------------------------------------------------------------------------- forward
 
+__device__ __forceinline__ void load_particle(const float* __restrict__ st, long LP, int p, bool with_F, ParticleIn& q)
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) q.x[a] = st[(SX + a) * LP + p];
+#pragma unroll
+    for (int a = 0; a < 3; a++) q.v[a] = st[(SV + a) * LP + p];
+#pragma unroll
+    for (int a = 0; a < 9; a++) q.C[a] = st[(SC + a) * LP + p];
+    if (with_F) {
+#pragma unroll
+        for (int a = 0; a < 9; a++) q.F[a] = st[(SF + a) * LP + p];
+    } else {
+#pragma unroll
+        for (int a = 0; a < 9; a++) q.F[a] = (a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f;
+    }
+}
+
 // P2G of simulation_functions.py:104-196 for all species of all instances.  Reads x, v, C, F
 // from `in`, writes F_tmp into `fout` (NULL: not written), scatters into grid (momentum xyz,
 // mass w) with float atomics.
@@ -62,66 +79,20 @@ __global__ void __launch_bounds__(256) k_p2g_ref(const float* __restrict__ in, f
     if (p >= P) return;
     int g = meta[p];
     if (g < 0) return;
-    Group G = groups[g];
+    const Group G = groups[g];
     const long LP = P;
-    float x[3], v[3], C[9];
-#pragma unroll
-    for (int a = 0; a < 3; a++) x[a] = in[(SX + a) * LP + p];
-#pragma unroll
-    for (int a = 0; a < 3; a++) v[a] = in[(SV + a) * LP + p];
-#pragma unroll
-    for (int a = 0; a < 9; a++) C[a] = in[(SC + a) * LP + p];
+    ParticleIn q;
+    load_particle(in, LP, p, G.species == LIQUID, q);
     Stencil s;
-    if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
-    float A[9], u[3], m;
-    if (G.species == LIQUID) {
-        float F[9];
-#pragma unroll
-        for (int a = 0; a < 9; a++) F[a] = in[(SF + a) * LP + p];
-        float th[4];
-        const double* pp = params + 4 * G.inst;
-#pragma unroll
-        for (int a = 0; a < 4; a++) th[a] = (float)pp[a];
-        double kb = sc.k_from_param ? pp[3] : sc.k_bulk;
-        LiquidP2G q;
-        liquid_p2g(sc, th, kb, C, F, q);
-        if (fout) {
-#pragma unroll
-            for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = q.Fn[a];
-        }
-#pragma unroll
-        for (int a = 0; a < 9; a++) A[a] = q.A[a];
-        m = sc.m_liq;
-        u[0] = v[0];
-        u[1] = v[1] + sc.gdt;
-        u[2] = v[2];
-    } else {
-        m = G.species == STICK ? sc.m_stick : sc.m_cont;
-#pragma unroll
-        for (int a = 0; a < 9; a++) A[a] = m * C[a];
-        u[0] = v[0]; u[1] = v[1]; u[2] = v[2];
-    }
+    if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
+    const InstParams ip = load_params(sc, params, G.inst);
     const int n = sc.n_grid;
-    float4* gi = grid + (long)G.inst * n * n * n;
-    float mu[3] = {m * u[0], m * u[1], m * u[2]};
+    ScatterGlobal4 acc{grid + (long)G.inst * n * n * n, n};
+    float Fn[9];
+    p2g_particle(sc, G.species, ip, q, s, Fn, acc);
+    if (fout && G.species == LIQUID) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        float d0 = ((float)i - s.fx[0]) * sc.dx;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            float d1 = ((float)j - s.fx[1]) * sc.dx;
-            float wij = s.w[0][i] * s.w[1][j];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                float d2 = ((float)k - s.fx[2]) * sc.dx;
-                float W = wij * s.w[2][k];
-                float4* node = gi + ((long)(s.b[0] + i) * n + (s.b[1] + j)) * n + (s.b[2] + k);
-                atomicAdd(&node->x, W * (mu[0] + A[0] * d0 + A[1] * d1 + A[2] * d2));
-                atomicAdd(&node->y, W * (mu[1] + A[3] * d0 + A[4] * d1 + A[5] * d2));
-                atomicAdd(&node->z, W * (mu[2] + A[6] * d0 + A[7] * d1 + A[8] * d2));
-                atomicAdd(&node->w, W * m);
-            }
-        }
+        for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
     }
 }
 
@@ -141,6 +112,37 @@ __global__ void __launch_bounds__(256) k_gridop_ref(const float4* __restrict__ g
     vout[id] = make_float4(v[0], v[1], v[2], 0.0f);
 }
 
+// Particle update after the gather (simulation_functions.py:246-290): liquid advects, the stick
+// moves kinematically and reports dv = v_kin - v_grid, the container keeps x and v = 0.
+__device__ __forceinline__ void g2p_write(float* __restrict__ out, long LP, int p, int sp, const float x[3],
+                                          const float nv[3], const float nC[9], const float* tr, float dt,
+                                          double dv[3])
+{
+    if (sp == LIQUID) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            out[(SV + a) * LP + p] = nv[a];
+            out[(SX + a) * LP + p] = x[a] + dt * nv[a];
+        }
+    } else if (sp == STICK) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float vk = (tr[3 + a] - tr[a]) / dt;
+            dv[a] = (double)(vk - nv[a]);
+            out[(SV + a) * LP + p] = vk;
+            out[(SX + a) * LP + p] = x[a] + vk * dt;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            out[(SV + a) * LP + p] = 0.0f;
+            out[(SX + a) * LP + p] = x[a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 9; a++) out[(SC + a) * LP + p] = nC[a];
+}
+
 // G2P of simulation_functions.py:228-290.  Reads x from `in`, writes x, v, C to `out`.  The
 // stick's dv = v_kin - v_grid is summed per instance into dvsum[inst][f+1] in FP64
 // (simulation_functions.py:266-267, 330-334); dvsum NULL skips it.
@@ -158,63 +160,17 @@ __global__ void __launch_bounds__(256) k_g2p_ref(const float* __restrict__ in, f
     double dv[3] = {0.0, 0.0, 0.0};
     const long LP = P;
     if (g >= 0) {
-        Group G = groups[g];
+        const Group G = groups[g];
         float x[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) x[a] = in[(SX + a) * LP + p];
         Stencil s;
         if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
         const int n = sc.n_grid;
-        const float4* vi = vout + (long)G.inst * n * n * n;
-        float nv[3] = {0.0f, 0.0f, 0.0f};
-        float nC[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            float d0 = (float)i - s.fx[0];
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                float d1 = (float)j - s.fx[1];
-                float wij = s.w[0][i] * s.w[1][j];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    float d2 = (float)k - s.fx[2];
-                    float W = wij * s.w[2][k];
-                    float4 gv = vi[((long)(s.b[0] + i) * n + (s.b[1] + j)) * n + (s.b[2] + k)];
-                    float wv[3] = {W * gv.x, W * gv.y, W * gv.z};
-#pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        nv[a] += wv[a];
-                        nC[a * 3 + 0] += sc.c4 * wv[a] * d0;
-                        nC[a * 3 + 1] += sc.c4 * wv[a] * d1;
-                        nC[a * 3 + 2] += sc.c4 * wv[a] * d2;
-                    }
-                }
-            }
-        }
-        if (G.species == LIQUID) {
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                out[(SV + a) * LP + p] = nv[a];
-                out[(SX + a) * LP + p] = x[a] + sc.dt * nv[a];
-            }
-        } else if (G.species == STICK) {
-            const float* tr = traj + ((long)G.inst * T + f) * 3;
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                float vk = (tr[3 + a] - tr[a]) / sc.dt;
-                dv[a] = (double)(vk - nv[a]);
-                out[(SV + a) * LP + p] = vk;
-                out[(SX + a) * LP + p] = x[a] + vk * sc.dt;
-            }
-        } else {
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                out[(SV + a) * LP + p] = 0.0f;
-                out[(SX + a) * LP + p] = x[a];
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 9; a++) out[(SC + a) * LP + p] = nC[a];
+        GatherVout acc{vout + (long)G.inst * n * n * n, n};
+        float nv[3], nC[9];
+        g2p_gather(sc, s, acc, nv, nC);
+        g2p_write(out, LP, p, G.species, x, nv, nC, traj + ((long)G.inst * T + f) * 3, sc.dt, dv);
     }
     if (dvsum && gw >= 0 && groups[gw].species == STICK) {
         double s0 = wave_sum(dv[0]), s1 = wave_sum(dv[1]), s2 = wave_sum(dv[2]);
@@ -243,7 +199,7 @@ __global__ void __launch_bounds__(256) k_g2p_adj_ref(const float* __restrict__ s
     if (p >= P) return;
     int g = meta[p];
     if (g < 0) return;
-    Group G = groups[g];
+    const Group G = groups[g];
     const long LP = P;
     float x[3];
 #pragma unroll
@@ -253,54 +209,27 @@ __global__ void __launch_bounds__(256) k_g2p_adj_ref(const float* __restrict__ s
     float Cb[9];
 #pragma unroll
     for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
-    float vt[3] = {0.0f, 0.0f, 0.0f};
-    bool liq = G.species == LIQUID;
-    if (liq) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) vt[a] = adj[(SV + a) * LP + p] + sc.dt * adj[(SX + a) * LP + p];
-    } else if (G.species == STICK) {
-        const double* d = dvbar + ((long)G.inst * T + f + 1) * 3;
-#pragma unroll
-        for (int a = 0; a < 3; a++) vt[a] = -(float)d[a];
-    }
     const int n = sc.n_grid;
-    const float4* vi = vout + (long)G.inst * n * n * n;
-    float4* bi = vbar + (long)G.inst * n * n * n;
-    float fxb[3] = {0.0f, 0.0f, 0.0f};
+    GatherVout gv{vout + (long)G.inst * n * n * n, n};
+    ScatterGlobal3 acc{vbar + (long)G.inst * n * n * n, n};
+    float vt[3] = {0.0f, 0.0f, 0.0f}, xb[3];
+    if (G.species == LIQUID) {
+        float xa[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
-                float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
-                long id = ((long)(s.b[0] + i) * n + (s.b[1] + j)) * n + (s.b[2] + k);
-                float cd[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) cd[a] = Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2];
-                atomicAdd(&bi[id].x, W * (vt[0] + sc.c4 * cd[0]));
-                atomicAdd(&bi[id].y, W * (vt[1] + sc.c4 * cd[1]));
-                atomicAdd(&bi[id].z, W * (vt[2] + sc.c4 * cd[2]));
-                if (liq) {
-                    float4 gv = vi[id];
-                    float gg[3] = {gv.x, gv.y, gv.z};
-                    float Wb = gg[0] * (vt[0] + sc.c4 * cd[0]) + gg[1] * (vt[1] + sc.c4 * cd[1])
-                             + gg[2] * (vt[2] + sc.c4 * cd[2]);
-                    float dW[3] = {s.dw[0][i] * s.w[1][j] * s.w[2][k], s.w[0][i] * s.dw[1][j] * s.w[2][k],
-                                   s.w[0][i] * s.w[1][j] * s.dw[2][k]};
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        float ctg = Cb[0 * 3 + c] * gg[0] + Cb[1 * 3 + c] * gg[1] + Cb[2 * 3 + c] * gg[2];
-                        fxb[c] += Wb * dW[c] - sc.c4 * W * ctg;
-                    }
-                }
-            }
+        for (int a = 0; a < 3; a++) {
+            xa[a] = adj[(SX + a) * LP + p];
+            vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
         }
-    }
-    if (liq) {
+        g2p_adj_particle<true>(sc, s, vt, Cb, gv, acc, xb);
 #pragma unroll
-        for (int a = 0; a < 3; a++) xbar[a * LP + p] = adj[(SX + a) * LP + p] + fxb[a] * sc.inv_dx;
+        for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
+    } else {
+        if (G.species == STICK) {
+            const double* d = dvbar + ((long)G.inst * T + f + 1) * 3;
+#pragma unroll
+            for (int a = 0; a < 3; a++) vt[a] = -(float)d[a];
+        }
+        g2p_adj_particle<false>(sc, s, vt, Cb, gv, acc, xb);
     }
 }
 
@@ -337,146 +266,39 @@ __global__ void __launch_bounds__(256) k_p2g_adj_ref(const float* __restrict__ s
     double tb[4] = {0.0, 0.0, 0.0, 0.0};
     const long LP = P;
     if (g >= 0) {
-        Group G = groups[g];
-        float x[3], C[9];
-#pragma unroll
-        for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
-#pragma unroll
-        for (int a = 0; a < 9; a++) C[a] = st[(SC + a) * LP + p];
-        Stencil s;
-        stencil(sc, x[0], x[1], x[2], s);
+        const Group G = groups[g];
         const int n = sc.n_grid;
-        const float4* pb = pbar + (long)G.inst * n * n * n;
+        GatherPbar acc{pbar + (long)G.inst * n * n * n, n};
         if (G.species == LIQUID) {
-            float v[3], F[9];
+            ParticleIn q;
+            load_particle(st, LP, p, true, q);
+            Stencil s;
+            stencil(sc, q.x[0], q.x[1], q.x[2], s);
+            const InstParams ip = load_params(sc, params, G.inst);
+            float Fnb[9], xp[3], xo[3], vo[3], Co[9], Fo[9];
 #pragma unroll
-            for (int a = 0; a < 3; a++) v[a] = st[(SV + a) * LP + p];
+            for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
 #pragma unroll
-            for (int a = 0; a < 9; a++) F[a] = st[(SF + a) * LP + p];
-            float th[4];
-            const double* pp = params + 4 * G.inst;
+            for (int a = 0; a < 3; a++) xp[a] = xbar[a * LP + p];
+            p2g_adj_liquid(sc, ip, q, s, Fnb, xp, acc, xo, vo, Co, Fo, tb);
 #pragma unroll
-            for (int a = 0; a < 4; a++) th[a] = (float)pp[a];
-            double kb = sc.k_from_param ? pp[3] : sc.k_bulk;
-            LiquidP2G q;
-            liquid_p2g(sc, th, kb, C, F, q);
-            const float m = sc.m_liq;
-            float u[3] = {v[0], v[1] + sc.gdt, v[2]};
-            float ub[3] = {0.0f, 0.0f, 0.0f}, Ab[9] = {0.0f}, fxb[3] = {0.0f, 0.0f, 0.0f};
+            for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
+            for (int a = 0; a < 3; a++) adj[(SV + a) * LP + p] = vo[a];
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
+            for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
 #pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        float dp[3] = {((float)i - s.fx[0]) * sc.dx, ((float)j - s.fx[1]) * sc.dx,
-                                       ((float)k - s.fx[2]) * sc.dx};
-                        float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
-                        float dW[3] = {s.dw[0][i] * s.w[1][j] * s.w[2][k], s.w[0][i] * s.dw[1][j] * s.w[2][k],
-                                       s.w[0][i] * s.w[1][j] * s.dw[2][k]};
-                        float4 gq = pb[((long)(s.b[0] + i) * n + (s.b[1] + j)) * n + (s.b[2] + k)];
-                        float gg[3] = {gq.x, gq.y, gq.z};
-                        float mom[3];
-#pragma unroll
-                        for (int a = 0; a < 3; a++)
-                            mom[a] = m * u[a] + q.A[a * 3] * dp[0] + q.A[a * 3 + 1] * dp[1] + q.A[a * 3 + 2] * dp[2];
-#pragma unroll
-                        for (int a = 0; a < 3; a++) {
-                            ub[a] += W * m * gg[a];
-#pragma unroll
-                            for (int c = 0; c < 3; c++) Ab[a * 3 + c] += W * gg[a] * dp[c];
-                        }
-                        float Wb = gg[0] * mom[0] + gg[1] * mom[1] + gg[2] * mom[2] + gq.w * m;
-#pragma unroll
-                        for (int c = 0; c < 3; c++) {
-                            float atg = q.A[0 * 3 + c] * gg[0] + q.A[1 * 3 + c] * gg[1] + q.A[2 * 3 + c] * gg[2];
-                            fxb[c] += Wb * dW[c] - sc.dx * W * atg;
-                        }
-                    }
-                }
-            }
-            float Cb[9], Sb[9];
-#pragma unroll
-            for (int a = 0; a < 9; a++) { Cb[a] = m * Ab[a]; Sb[a] = sc.alpha * Ab[a]; }
-            float etab = 0.0f;
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) etab += Sb[a * 3 + c] * 0.5f * (C[a * 3 + c] + C[c * 3 + a]);
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) Cb[a * 3 + c] += q.eta * 0.5f * (Sb[a * 3 + c] + Sb[c * 3 + a]);
-            double trS = (double)Sb[0] + (double)Sb[4] + (double)Sb[8];
-            double J = q.J;
-            double Jb = trS * (kb * 0.5) * (2.0 * J - 1.0);
-            if (sc.k_from_param) tb[3] += trS * 0.5 * (J - 1.0 / J - 1.0) * J;
-            Visc vv = viscosity(sc.model, th, q.gamma);
-#pragma unroll
-            for (int a = 0; a < 4; a++) tb[a] += (double)etab * (double)vv.d_dth[a];
-            float gb = etab * vv.d_dg;
-            float nrm = q.gamma - 1e-6f;
-            float n2 = 0.0f;
-#pragma unroll
-            for (int a = 0; a < 9; a++) n2 = fmaf(C[a], C[a], n2);
-            if (n2 > 0.0f) {
-                float inr = 1.0f / sqrtf(n2);
-                (void)nrm;
-#pragma unroll
-                for (int a = 0; a < 9; a++) Cb[a] += gb * C[a] * inr;
-            }
-            // J = S0 S1 S2: dJ/dS_i = prod of the other two; back through the SVD.
-            double gS[3] = {Jb * q.S[1] * q.S[2], Jb * q.S[0] * q.S[2], Jb * q.S[0] * q.S[1]};
-            double gFn[9];
-            svd3_backward_sigma(q.U, q.V, gS, gFn);
-            float Fnb[9];
-#pragma unroll
-            for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p] + (float)gFn[a];
-            // F_tmp = (I + dt C) F
-            float Fb[9];
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    float sF = 0.0f, sC2 = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        float Mka = (k == a ? 1.0f : 0.0f) + sc.dt * C[k * 3 + a];
-                        sF += Mka * Fnb[k * 3 + c];
-                        sC2 += Fnb[a * 3 + k] * F[c * 3 + k];
-                    }
-                    Fb[a * 3 + c] = sF;
-                    Cb[a * 3 + c] += sc.dt * sC2;
-                }
-#pragma unroll
-            for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xbar[a * LP + p] + fxb[a] * sc.inv_dx;
-#pragma unroll
-            for (int a = 0; a < 3; a++) adj[(SV + a) * LP + p] = ub[a];
-#pragma unroll
-            for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Cb[a];
-#pragma unroll
-            for (int a = 0; a < 9; a++) adj[(SF + a) * LP + p] = Fb[a];
+            for (int a = 0; a < 9; a++) adj[(SF + a) * LP + p] = Fo[a];
         } else {
-            const float m = G.species == STICK ? sc.m_stick : sc.m_cont;
-            float Cb[9] = {0.0f};
+            float x[3];
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+            for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
+            Stencil s;
+            stencil(sc, x[0], x[1], x[2], s);
+            float Co[9];
+            p2g_adj_rigid(sc, species_mass(sc, G.species), s, acc, Co);
 #pragma unroll
-                for (int j = 0; j < 3; j++)
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        float dp[3] = {((float)i - s.fx[0]) * sc.dx, ((float)j - s.fx[1]) * sc.dx,
-                                       ((float)k - s.fx[2]) * sc.dx};
-                        float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
-                        float4 gq = pb[((long)(s.b[0] + i) * n + (s.b[1] + j)) * n + (s.b[2] + k)];
-                        float gg[3] = {gq.x, gq.y, gq.z};
-#pragma unroll
-                        for (int a = 0; a < 3; a++)
-#pragma unroll
-                            for (int c = 0; c < 3; c++) Cb[a * 3 + c] += W * gg[a] * dp[c];
-                    }
-#pragma unroll
-            for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = m * Cb[a];
+            for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
         }
     }
     if (gw >= 0 && groups[gw].species == LIQUID) {
