@@ -69,13 +69,13 @@ def grad_eval(eng: Engine, targets, kind):
     return losses, d
 
 
-def run_gpu(workload: str, steps: int, warmup: int, ckpt: int):
+def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled"):
     scenes = S.workload(workload)
     n_liq = sum(s.n_liquid for s in scenes)
     T = scenes[0].max_steps
     log(f"[bench] workload {workload}: {len(scenes)} instance(s), {n_liq} liquid particles, {T - 1} substeps, "
         f"checkpoint every {ckpt}")
-    eng = Engine(scenes, ckpt_every=ckpt)
+    eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels)
     targets = make_targets(eng, scenes)
     eng.set_params(candidate_params(scenes))
     kind = scenes[0].cfg.loss_kind
@@ -128,12 +128,13 @@ def main():
     ap.add_argument("--ckpt", type=int, default=50)
     ap.add_argument("--cpu-sample-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels", default="tiled", choices=["tiled", "per_particle"])
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the instances of a batch share one device "
               "and nothing here uses a second GPU", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt)
+    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels)
     value = r["psteps"] / r["elapsed"]
     ms_per_step = 1000.0 * r["elapsed"] / a.steps
     bpps = fwd_adj_bytes(a.ckpt)
@@ -152,7 +153,8 @@ def main():
         "dtype": "f32 (FP64 SVD, SVD backward, loss and gradient accumulation)",
         "data": "synthetic (seeded cylinder fill, circular stick path)",
         "config": {"workload": a.workload, "instances": r["B"], "liquid_particles": r["n_liq"],
-                   "substeps": r["T"] - 1, "n_grid": r["scenes"][0].cfg.n_grid, "ckpt_every": a.ckpt},
+                   "substeps": r["T"] - 1, "n_grid": r["scenes"][0].cfg.n_grid, "ckpt_every": a.ckpt,
+                   "kernels": a.kernels},
         "extra": {"gradient_evals_per_s": r["B"] * a.steps / r["elapsed"], "fwd_ms": r["fwd_ms"],
                   "adjoint_ms": r["bwd_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libdiffstir_hip.so")
 
 # the exported entry points of include/diffstir.h
 EXPORTS = (
-    "ds_create", "ds_destroy", "ds_last_error", "ds_set_particles", "ds_set_trajectory", "ds_set_params",
+    "ds_create", "ds_destroy", "ds_set_option", "ds_last_error", "ds_set_particles", "ds_set_trajectory", "ds_set_params",
     "ds_reset", "ds_forward", "ds_current_step", "ds_get_forces", "ds_backward", "ds_read_liquid",
     "ds_read_stick", "ds_synchronize", "ds_last_timing", "ds_svd3", "ds_svd3_backward",
 )
@@ -62,6 +62,7 @@ def load() -> C.CDLL:
         "ds_create": (C.c_int, [C.POINTER(SceneDesc), C.c_int32, I32P, I32P, I32P, C.c_int32, C.c_int32,
                                 C.POINTER(vp)]),
         "ds_destroy": (None, [vp]),
+        "ds_set_option": (C.c_int, [vp, C.c_int32, C.c_int32]),
         "ds_last_error": (C.c_char_p, [vp]),
         "ds_set_particles": (C.c_int, [vp, C.c_int32, F32P, F32P, F32P]),
         "ds_set_trajectory": (C.c_int, [vp, C.c_int32, F32P]),

@@ -3,15 +3,19 @@
 // One translation unit: the kernel files are included here so that every kernel and the
 // host code that launches it are compiled together (no relocatable device code).
 //
-// Device memory of a context (P = particles of all instances and species, each
-// (instance, species) run padded to a multiple of 64; G = n_grid^3 nodes per instance):
-//   state      24 x P float   current x, v, C, F (component-major SoA)
-//   ckpt       nck x 24 x P   checkpoint every ckpt_every substeps
-//   seg        K x 24 x P     states of one segment during the adjoint recompute
-//   adj, xbar  24 x P, 3 x P  adjoint state, partial dL/dx
-//   grid/vout/vbar/pbar       4 x B x G float4 (momentum+mass, velocity, their adjoints)
-//   dvsum/dvbar               B x T x 3 double (stick sum of dv and its adjoint)
-//   params/thbar              B x 4 double
+// Device memory of a context (P = particle slots of all instances and species: each
+// (instance, species) run padded to a multiple of 64 in the initial layout; G = n_grid^3
+// nodes per instance, B instances, K = checkpoint interval):
+//   state           24 x P float   current x, v, C, F (component-major SoA)
+//   ckpt            nck x 24 x P   state at every K-th substep (sorted order in tiled mode)
+//   seg             K x 24 x P     states of one segment during the adjoint recompute
+//   adj, adj2, xbar 24 x P, 24 x P, 3 x P   adjoint state (+ re-sort buffer), partial dL/dx
+//   dvsum/dvbar     B x T x 3 double (stick sum of dv per step and its adjoint)
+//   params/thbar    B x 4 double
+// per-particle kernels:  grid, vout, vbar, pbar       4 x B x G float4
+// tiled kernels:         grid[2], vbar[2]             4 x B x G float4 (double buffers)
+//                        seg_grid                     K x B x G float4 (momentum/mass per step)
+//                        bucket sort scratch, chunk tables and re-sort maps per checkpoint
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,8 +26,9 @@
 #include <vector>
 
 #include "../../include/diffstir.h"
-#include "ds_device.h"
+#include "ds_particle.h"
 #include "kernels_ref.hip"
+#include "kernels_tiled.hip"
 
 using namespace ds;
 
@@ -31,16 +36,19 @@ struct ds_ctx {
     ds_scene_desc desc;
     DevScene dsc;
     int B = 0, T = 0, K = 0, ncp = 0;
-    int P = 0;
+    int P = 0, nreal = 0;
+    int mode = DS_KERNELS_TILED;       // requested
+    int alloc_mode = -1;               // mode whose buffers are allocated
     std::vector<int> nl, ns, nc;
     std::vector<Group> groups;
-    std::vector<int> meta_h;
-    std::vector<float> x0_h;      // [P][3] initial positions in particle order
+    std::vector<int> meta_h, pid_h;
+    std::vector<float> x0_h;      // [P][3] initial positions in the initial layout
     std::vector<float> traj_h;    // [B][T][3]
     std::vector<double> params_h; // [B][4]
     int step = 0;
     bool reset_done = false;
 
+    // common
     int* d_meta = nullptr;
     Group* d_groups = nullptr;
     float* d_x0 = nullptr;
@@ -49,16 +57,40 @@ struct ds_ctx {
     float* d_seg = nullptr;
     float* d_adj = nullptr;
     float* d_xbar = nullptr;
-    float4* d_grid = nullptr;
-    float4* d_vout = nullptr;
-    float4* d_vbar = nullptr;
-    float4* d_pbar = nullptr;
     double* d_dvsum = nullptr;
     double* d_dvbar = nullptr;
     double* d_params = nullptr;
     double* d_thbar = nullptr;
     float* d_traj = nullptr;
     unsigned* d_flag = nullptr;
+    // per-particle kernels
+    float4* d_grid = nullptr;
+    float4* d_vout = nullptr;
+    float4* d_vbar = nullptr;
+    float4* d_pbar = nullptr;
+    // tiled kernels
+    float4* d_g2[2] = {nullptr, nullptr};
+    float4* d_vb2[2] = {nullptr, nullptr};
+    float4* d_seg_grid = nullptr;
+    float* d_adj2 = nullptr;
+    int* d_meta2 = nullptr;
+    int* d_pid = nullptr;
+    int* d_pid2 = nullptr;
+    int* d_cnt = nullptr;
+    int* d_celloff = nullptr;
+    int* d_bcount = nullptr;
+    int* d_bnch = nullptr;
+    int* d_bstart = nullptr;
+    int* d_cstart = nullptr;
+    int* d_tot = nullptr;
+    int* d_key = nullptr;
+    int* d_rank = nullptr;
+    int* d_rrpos = nullptr;
+    Chunk* d_chunks = nullptr;    // ncp x maxch
+    int* d_src = nullptr;         // ncp x P
+    int NB = 0, NBK = 0;
+    long maxch = 0;
+    std::vector<int> nch_h;       // chunks per checkpoint
 
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -87,6 +119,12 @@ int fail(ds_ctx* c, int code, const std::string& msg)
             return fail(ctx, DS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
+#define RCHK(expr)                \
+    do {                          \
+        int r_ = (expr);          \
+        if (r_ != DS_OK) return r_; \
+    } while (0)
+
 template <typename T>
 int dalloc(ds_ctx* c, T** p, size_t count)
 {
@@ -94,33 +132,44 @@ int dalloc(ds_ctx* c, T** p, size_t count)
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError();
         return fail(c, DS_ENOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
                                       " bytes failed: " + hipGetErrorString(e));
     }
     return DS_OK;
 }
 
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
 inline unsigned nblocks(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
-int fill_zero_f4(ds_ctx* c, float4* p, long n)
+void fill_zero_f4(ds_ctx* c, float4* p, long n)
 {
-    unsigned nb = std::min<long>(nblocks(n), 8192);
+    unsigned nb = (unsigned)std::min<long>(nblocks(n), 8192);
     hipLaunchKernelGGL(k_fill_f4, dim3(nb), dim3(256), 0, c->stream, p, n);
-    return DS_OK;
 }
 
-int fill_zero_f32(ds_ctx* c, float* p, long n)
+void fill_zero_f32(ds_ctx* c, float* p, long n)
 {
-    unsigned nb = std::min<long>(nblocks(n), 8192);
+    unsigned nb = (unsigned)std::min<long>(nblocks(n), 8192);
     hipLaunchKernelGGL(k_fill_f32, dim3(nb), dim3(256), 0, c->stream, p, n, 0.0f);
-    return DS_OK;
 }
 
-int fill_zero_f64(ds_ctx* c, double* p, long n)
+void fill_zero_f64(ds_ctx* c, double* p, long n)
 {
-    unsigned nb = std::min<long>(nblocks(n), 8192);
+    unsigned nb = (unsigned)std::min<long>(nblocks(n), 8192);
     hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(256), 0, c->stream, p, n);
-    return DS_OK;
+}
+
+void fill_i32(ds_ctx* c, int* p, long n, int v)
+{
+    unsigned nb = (unsigned)std::min<long>(nblocks(n), 8192);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, c->stream, p, n, v);
 }
 
 DevScene make_dev_scene(const ds_scene_desc& d)
@@ -162,6 +211,114 @@ int check_flag(ds_ctx* c, const char* where)
     return DS_OK;
 }
 
+void free_mode(ds_ctx* c)
+{
+    dfree(c->d_seg);
+    dfree(c->d_grid);
+    dfree(c->d_vout);
+    dfree(c->d_vbar);
+    dfree(c->d_pbar);
+    dfree(c->d_g2[0]);
+    dfree(c->d_g2[1]);
+    dfree(c->d_vb2[0]);
+    dfree(c->d_vb2[1]);
+    dfree(c->d_seg_grid);
+    dfree(c->d_adj2);
+    dfree(c->d_meta2);
+    dfree(c->d_pid);
+    dfree(c->d_pid2);
+    dfree(c->d_cnt);
+    dfree(c->d_celloff);
+    dfree(c->d_bcount);
+    dfree(c->d_bnch);
+    dfree(c->d_bstart);
+    dfree(c->d_cstart);
+    dfree(c->d_tot);
+    dfree(c->d_key);
+    dfree(c->d_rank);
+    dfree(c->d_rrpos);
+    dfree(c->d_chunks);
+    dfree(c->d_src);
+    c->alloc_mode = -1;
+}
+
+void free_all(ds_ctx* c)
+{
+    free_mode(c);
+    dfree(c->d_meta);
+    dfree(c->d_groups);
+    dfree(c->d_x0);
+    dfree(c->d_state);
+    dfree(c->d_ckpt);
+    dfree(c->d_adj);
+    dfree(c->d_xbar);
+    dfree(c->d_dvsum);
+    dfree(c->d_dvbar);
+    dfree(c->d_params);
+    dfree(c->d_thbar);
+    dfree(c->d_traj);
+    dfree(c->d_flag);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->ev0 = c->ev1 = nullptr;
+    c->stream = nullptr;
+}
+
+// Buffers that depend on the kernel family, allocated at the first ds_reset after a change.
+int alloc_mode(ds_ctx* c)
+{
+    if (c->alloc_mode == c->mode) return DS_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_mode(c);
+    const long G = c->gridNodes() * c->B;
+    const size_t SF = (size_t)c->stateFloats();
+    int rc;
+#define MALLOC(ptr, n)                                  \
+    if ((rc = dalloc(c, &ptr, (size_t)(n))) != DS_OK) { \
+        free_mode(c);                                   \
+        return rc;                                      \
+    }
+    MALLOC(c->d_seg, SF * c->K);
+    if (c->mode == DS_KERNELS_PER_PARTICLE) {
+        MALLOC(c->d_grid, G);
+        MALLOC(c->d_vout, G);
+        MALLOC(c->d_vbar, G);
+        MALLOC(c->d_pbar, G);
+    } else {
+        c->NB = (c->desc.n_grid + TB - 1) / TB;
+        c->NBK = c->B * 3 * c->NB * c->NB * c->NB;
+        c->maxch = (long)c->P / 64 + std::min<long>(c->P, c->NBK) + 1;
+        MALLOC(c->d_g2[0], G);
+        MALLOC(c->d_g2[1], G);
+        MALLOC(c->d_vb2[0], G);
+        MALLOC(c->d_vb2[1], G);
+        MALLOC(c->d_seg_grid, (size_t)G * c->K);
+        MALLOC(c->d_adj2, SF);
+        MALLOC(c->d_meta2, c->P);
+        MALLOC(c->d_pid, c->P);
+        MALLOC(c->d_pid2, c->P);
+        MALLOC(c->d_cnt, (size_t)c->NBK * 64);
+        MALLOC(c->d_celloff, (size_t)c->NBK * 64);
+        MALLOC(c->d_bcount, c->NBK);
+        MALLOC(c->d_bnch, c->NBK);
+        MALLOC(c->d_bstart, c->NBK);
+        MALLOC(c->d_cstart, c->NBK);
+        MALLOC(c->d_tot, 2);
+        MALLOC(c->d_key, c->P);
+        MALLOC(c->d_rank, c->P);
+        MALLOC(c->d_rrpos, c->P);
+        MALLOC(c->d_chunks, (size_t)c->maxch * c->ncp);
+        MALLOC(c->d_src, (size_t)c->P * c->ncp);
+        c->nch_h.assign(c->ncp, 0);
+    }
+#undef MALLOC
+    c->alloc_mode = c->mode;
+    return DS_OK;
+}
+
+// ---------------------------------------------------------------------------- per-particle path
+
 // One forward substep with the per-particle kernels: state `in` at step f -> `out` at f+1.
 void step_ref(ds_ctx* c, const float* in, float* out, int f, bool record)
 {
@@ -193,16 +350,185 @@ void step_adj_ref(ds_ctx* c, const float* st, int f)
                        c->d_meta, c->d_groups, c->d_params, c->d_pbar, c->d_thbar, c->dsc, c->P);
 }
 
-void free_all(ds_ctx* c)
+int forward_ref(ds_ctx* c, int nsteps)
 {
-    void* ptrs[] = {c->d_meta, c->d_groups, c->d_x0, c->d_state, c->d_ckpt, c->d_seg, c->d_adj, c->d_xbar,
-                    c->d_grid, c->d_vout, c->d_vbar, c->d_pbar, c->d_dvsum, c->d_dvbar, c->d_params,
-                    c->d_thbar, c->d_traj, c->d_flag};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    const long SF = c->stateFloats();
+    for (int i = 0; i < nsteps; i++) {
+        const int s = c->step;
+        if (s % c->K == 0)
+            HIPCHK(c, hipMemcpyAsync(c->d_ckpt + (size_t)(s / c->K) * SF, c->d_state, sizeof(float) * SF,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        step_ref(c, c->d_state, c->d_state, s, true);
+        c->step++;
+    }
+    return DS_OK;
+}
+
+int backward_ref(ds_ctx* c)
+{
+    const int S = c->step;
+    const long SF = c->stateFloats();
+    for (int cp = (S - 1) / c->K; cp >= 0; cp--) {
+        const int s0 = cp * c->K;
+        const int s1 = std::min(s0 + c->K, S);
+        HIPCHK(c, hipMemcpyAsync(c->d_seg, c->d_ckpt + (size_t)cp * SF, sizeof(float) * SF, hipMemcpyDeviceToDevice,
+                                 c->stream));
+        for (int s = s0; s < s1 - 1; s++) {
+            const int k = s - s0;
+            step_ref(c, c->d_seg + (size_t)k * SF, c->d_seg + (size_t)(k + 1) * SF, s, false);
+        }
+        for (int s = s1 - 1; s >= s0; s--) step_adj_ref(c, c->d_seg + (size_t)(s - s0) * SF, s);
+    }
+    return DS_OK;
+}
+
+// ---------------------------------------------------------------------------- tiled path
+
+// Sort the current state into checkpoint slot cp (bucket order, round-robin within buckets),
+// build that checkpoint's chunk table and record where every slot came from.
+int sort_into_checkpoint(ds_ctx* c, int cp)
+{
+    const long SF = c->stateFloats();
+    float* dst = c->d_ckpt + (size_t)cp * SF;
+    Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
+    int* src = c->d_src + (size_t)cp * c->P;
+    fill_i32(c, c->d_cnt, (long)c->NBK * 64, 0);
+    hipLaunchKernelGGL(k_sort_count, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_meta,
+                       c->d_groups, c->d_key, c->d_rank, c->d_cnt, c->dsc, c->NB, c->P);
+    hipLaunchKernelGGL(k_bucket_totals, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
+                       c->d_celloff, c->d_bcount, c->d_bnch, c->NBK, WG);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, c->stream, c->d_bcount, c->d_bnch, c->d_bstart, c->d_cstart,
+                       c->NBK, c->d_tot);
+    hipLaunchKernelGGL(k_bucket_layout, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
+                       c->d_celloff, c->d_bstart, c->d_cstart, c->d_rrpos, chunks, c->NB, c->NBK, WG);
+    fill_i32(c, c->d_meta2, c->P, -1);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
+                       c->d_rank, c->d_celloff, c->d_bstart, c->d_rrpos, c->d_meta, c->d_meta2, c->d_pid, c->d_pid2,
+                       src, c->P);
+    std::swap(c->d_meta, c->d_meta2);
+    std::swap(c->d_pid, c->d_pid2);
+    int tot[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(tot, c->d_tot, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (tot[0] != c->nreal || tot[1] < 0 || tot[1] > c->maxch)
+        return fail(c, DS_EHIP, "bucket sort lost particles (" + std::to_string(tot[0]) + " of " +
+                                    std::to_string(c->nreal) + ")");
+    c->nch_h[cp] = tot[1];
+    return DS_OK;
+}
+
+void p2g_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* grid)
+{
+    const int nch = c->nch_h[cp];
+    if (nch == 0) return;
+    hipLaunchKernelGGL(k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout,
+                       c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, grid, c->dsc, c->P, c->d_flag);
+}
+
+void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, const float4* grid, float4* clear_next, int f,
+               bool record)
+{
+    const int nch = std::max(c->nch_h[cp], 1);
+    hipLaunchKernelGGL(k_g2p_tiled, dim3(nch), dim3(WG), 0, c->stream, in, out, c->d_chunks + (size_t)cp * c->maxch,
+                       c->nch_h[cp], grid, clear_next, c->gridNodes() * c->B, c->d_traj, c->T, f,
+                       record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
+}
+
+int forward_tiled(ds_ctx* c, int nsteps)
+{
+    const long SF = c->stateFloats();
+    const long G = c->gridNodes() * c->B;
+    for (int i = 0; i < nsteps; i++) {
+        const int s = c->step;
+        const int cp = s / c->K;
+        const float* in = c->d_state;
+        if (s % c->K == 0) {
+            RCHK(sort_into_checkpoint(c, cp));
+            in = c->d_ckpt + (size_t)cp * SF;
+        }
+        float4* g = c->d_g2[s & 1];
+        p2g_tiled(c, in, c->d_state, cp, g);
+        g2p_tiled(c, in, c->d_state, cp, g, c->d_g2[(s + 1) & 1], s, true);
+        (void)G;
+        c->step++;
+    }
+    return DS_OK;
+}
+
+int backward_tiled(ds_ctx* c)
+{
+    const int S = c->step;
+    const long SF = c->stateFloats();
+    const long G = c->gridNodes() * c->B;
+    fill_zero_f4(c, c->d_vb2[0], G);
+    fill_zero_f4(c, c->d_vb2[1], G);
+    for (int cp = (S - 1) / c->K; cp >= 0; cp--) {
+        const int s0 = cp * c->K;
+        const int s1 = std::min(s0 + c->K, S);
+        const int L = s1 - s0;
+        const int nch = c->nch_h[cp];
+        const Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
+        const float* ck = c->d_ckpt + (size_t)cp * SF;
+        auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : c->d_seg + (size_t)k * SF; };
+        auto grid_of = [&](int k) -> float4* { return c->d_seg_grid + (size_t)k * G; };
+        // recompute the segment, keeping each step's state and momentum/mass grid
+        fill_zero_f4(c, grid_of(0), G);
+        for (int k = 0; k < L; k++) {
+            const int s = s0 + k;
+            p2g_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k));
+            if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, grid_of(k), grid_of(k + 1), s, false);
+        }
+        // adjoint, last step first
+        if (nch > 0) {
+            for (int k = L - 1; k >= 0; k--) {
+                const int s = s0 + k;
+                hipLaunchKernelGGL(k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj, chunks, nch,
+                                   grid_of(k), c->d_dvbar, c->T, s, c->d_vb2[s & 1], c->d_xbar, c->dsc, c->P);
+                hipLaunchKernelGGL(k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj, c->d_xbar,
+                                   chunks, nch, c->d_params, grid_of(k), c->d_vb2[s & 1], c->d_vb2[(s + 1) & 1], G,
+                                   c->d_thbar, c->dsc, c->P);
+            }
+        }
+        if (cp > 0) {
+            hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
+                               c->d_src + (size_t)cp * c->P, c->nreal, c->P);
+            std::swap(c->d_adj, c->d_adj2);
+        }
+    }
+    return DS_OK;
+}
+
+// Copy one species of one instance out of the current state, in the order it was given.
+int read_species(ds_ctx* c, int inst, int sp, int slot0, int ncomp, float* out)
+{
+    if (!out) return DS_OK;
+    const Group& g = c->groups[inst * 3 + sp];
+    if (g.count == 0) return DS_OK;
+    if (c->alloc_mode != DS_KERNELS_TILED) {
+        std::vector<float> tmp((size_t)ncomp * g.count);
+        for (int a = 0; a < ncomp; a++)
+            HIPCHK(c, hipMemcpyAsync(tmp.data() + (size_t)a * g.count, c->d_state + (size_t)(slot0 + a) * c->P + g.offset,
+                                     sizeof(float) * g.count, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < g.count; i++)
+            for (int a = 0; a < ncomp; a++) out[(size_t)i * ncomp + a] = tmp[(size_t)a * g.count + i];
+        return DS_OK;
+    }
+    std::vector<int> meta(c->P), pid(c->P);
+    std::vector<float> tmp((size_t)ncomp * c->P);
+    HIPCHK(c, hipMemcpyAsync(meta.data(), c->d_meta, sizeof(int) * c->P, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pid.data(), c->d_pid, sizeof(int) * c->P, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->d_state + (size_t)slot0 * c->P, sizeof(float) * ncomp * c->P,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int gid = inst * 3 + sp;
+    for (int p = 0; p < c->P; p++) {
+        if (meta[p] != gid) continue;
+        const int i = pid[p];
+        if (i < 0 || i >= g.count) return fail(c, DS_EHIP, "particle index table is corrupt");
+        for (int a = 0; a < ncomp; a++) out[(size_t)i * ncomp + a] = tmp[(size_t)a * c->P + p];
+    }
+    return DS_OK;
 }
 
 }  // namespace
@@ -231,9 +557,9 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
     c->T = max_steps;
     c->K = ckpt_every;
     c->ncp = (max_steps - 2) / ckpt_every + 1;
-    long P = 0;
+    long P = 0, nreal = 0;
     for (int b = 0; b < n_inst; b++) {
-        int cnt[3] = {n_liquid[b], n_stick[b], n_container[b]};
+        const int cnt[3] = {n_liquid[b], n_stick[b], n_container[b]};
         c->nl.push_back(cnt[0]);
         c->ns.push_back(cnt[1]);
         c->nc.push_back(cnt[2]);
@@ -242,9 +568,9 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
                 delete c;
                 return fail(nullptr, DS_EINVAL, "negative particle count");
             }
-            Group g{b, sp, (int)P, cnt[sp]};
-            c->groups.push_back(g);
+            c->groups.push_back(Group{b, sp, (int)P, cnt[sp]});
             P += ((long)cnt[sp] + 63) / 64 * 64;
+            nreal += cnt[sp];
         }
     }
     if (P < 64) P = 64;
@@ -253,34 +579,33 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
         return fail(nullptr, DS_EINVAL, "too many particles");
     }
     c->P = (int)P;
+    c->nreal = (int)nreal;
     c->meta_h.assign(P, -1);
+    c->pid_h.assign(P, -1);
     for (size_t g = 0; g < c->groups.size(); g++)
-        for (int i = 0; i < c->groups[g].count; i++) c->meta_h[c->groups[g].offset + i] = (int)g;
+        for (int i = 0; i < c->groups[g].count; i++) {
+            c->meta_h[c->groups[g].offset + i] = (int)g;
+            c->pid_h[c->groups[g].offset + i] = i;
+        }
     c->x0_h.assign(3 * P, 0.5f);
     c->traj_h.assign((size_t)c->B * c->T * 3, 0.0f);
     c->params_h.assign((size_t)c->B * 4, 0.0);
 
     int rc;
-    const long G = c->gridNodes();
-#define ALLOC(ptr, n)                      \
-    if ((rc = dalloc(c, &ptr, (n))) != 0) { \
-        std::string m = c->err;            \
-        free_all(c);                       \
-        delete c;                          \
-        return fail(nullptr, rc, m);       \
+#define ALLOC(ptr, n)                          \
+    if ((rc = dalloc(c, &ptr, (n))) != 0) {     \
+        std::string m = c->err;                \
+        free_all(c);                           \
+        delete c;                              \
+        return fail(nullptr, rc, m);           \
     }
     ALLOC(c->d_meta, (size_t)P);
     ALLOC(c->d_groups, c->groups.size());
     ALLOC(c->d_x0, (size_t)3 * P);
     ALLOC(c->d_state, (size_t)c->stateFloats());
     ALLOC(c->d_ckpt, (size_t)c->stateFloats() * c->ncp);
-    ALLOC(c->d_seg, (size_t)c->stateFloats() * c->K);
     ALLOC(c->d_adj, (size_t)c->stateFloats());
     ALLOC(c->d_xbar, (size_t)3 * P);
-    ALLOC(c->d_grid, (size_t)G * c->B);
-    ALLOC(c->d_vout, (size_t)G * c->B);
-    ALLOC(c->d_vbar, (size_t)G * c->B);
-    ALLOC(c->d_pbar, (size_t)G * c->B);
     ALLOC(c->d_dvsum, (size_t)c->B * c->T * 3);
     ALLOC(c->d_dvbar, (size_t)c->B * c->T * 3);
     ALLOC(c->d_params, (size_t)c->B * 4);
@@ -291,7 +616,6 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
     hipError_t e = hipStreamCreate(&c->stream);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-    if (e == hipSuccess) e = hipMemcpy(c->d_meta, c->meta_h.data(), sizeof(int) * P, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c->d_groups, c->groups.data(), sizeof(Group) * c->groups.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(c->d_flag, 0, sizeof(unsigned));
@@ -311,6 +635,19 @@ void ds_destroy(ds_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_all(ctx);
     delete ctx;
+}
+
+int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
+{
+    if (!c) return DS_EINVAL;
+    if (option == DS_OPT_KERNELS) {
+        if (value != DS_KERNELS_TILED && value != DS_KERNELS_PER_PARTICLE)
+            return fail(c, DS_EINVAL, "DS_OPT_KERNELS takes DS_KERNELS_TILED or DS_KERNELS_PER_PARTICLE");
+        if (value != c->mode) c->reset_done = false;
+        c->mode = value;
+        return DS_OK;
+    }
+    return fail(c, DS_EINVAL, "unknown option");
 }
 
 const char* ds_last_error(const ds_ctx* ctx)
@@ -360,7 +697,13 @@ int ds_set_params(ds_ctx* c, const double* params)
 int ds_reset(ds_ctx* c)
 {
     if (!c) return DS_EINVAL;
+    RCHK(alloc_mode(c));
     HIPCHK(c, hipMemcpyAsync(c->d_x0, c->x0_h.data(), sizeof(float) * 3 * c->P, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_meta, c->meta_h.data(), sizeof(int) * c->P, hipMemcpyHostToDevice, c->stream));
+    if (c->mode == DS_KERNELS_TILED) {
+        HIPCHK(c, hipMemcpyAsync(c->d_pid, c->pid_h.data(), sizeof(int) * c->P, hipMemcpyHostToDevice, c->stream));
+        fill_zero_f4(c, c->d_g2[0], c->gridNodes() * c->B);
+    }
     hipLaunchKernelGGL(k_init_state, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_x0, c->d_meta,
                        c->P, 0.5f);
     fill_zero_f64(c, c->d_dvsum, (long)c->B * c->T * 3);
@@ -378,15 +721,8 @@ int ds_forward(ds_ctx* c, int32_t nsteps)
     if (nsteps < 0 || c->step + nsteps > c->T - 1)
         return fail(c, DS_EINVAL, "nsteps exceeds the trajectory (max_steps - 1 substeps in total)");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    const long SF = c->stateFloats();
-    for (int i = 0; i < nsteps; i++) {
-        int s = c->step;
-        if (s % c->K == 0)
-            HIPCHK(c, hipMemcpyAsync(c->d_ckpt + (size_t)(s / c->K) * SF, c->d_state, sizeof(float) * SF,
-                                     hipMemcpyDeviceToDevice, c->stream));
-        step_ref(c, c->d_state, c->d_state, s, true);
-        c->step++;
-    }
+    if (c->mode == DS_KERNELS_TILED) RCHK(forward_tiled(c, nsteps));
+    else RCHK(forward_ref(c, nsteps));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -407,10 +743,10 @@ int ds_get_forces(ds_ctx* c, double* forces)
 {
     if (!c) return DS_EINVAL;
     if (!forces) return fail(c, DS_EINVAL, "NULL forces");
-    size_t n = (size_t)c->B * c->T * 3;
+    const size_t n = (size_t)c->B * c->T * 3;
     HIPCHK(c, hipMemcpyAsync(forces, c->d_dvsum, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    double scale = c->desc.m_stick / c->desc.dt;
+    const double scale = c->desc.m_stick / c->desc.dt;
     for (size_t i = 0; i < n; i++) forces[i] *= scale;
     return DS_OK;
 }
@@ -420,27 +756,16 @@ int ds_backward(ds_ctx* c, const double* dl_dforces, double* dl_dparams)
     if (!c) return DS_EINVAL;
     if (!dl_dforces || !dl_dparams) return fail(c, DS_EINVAL, "NULL argument");
     if (!c->reset_done || c->step < 1) return fail(c, DS_ESTATE, "ds_backward needs a recorded forward rollout");
-    const int S = c->step;
     // dL/d(dv sum at step s) = m_stick/dt * dL/dF(s)
     std::vector<double> dvb((size_t)c->B * c->T * 3);
-    double scale = c->desc.m_stick / c->desc.dt;
+    const double scale = c->desc.m_stick / c->desc.dt;
     for (size_t i = 0; i < dvb.size(); i++) dvb[i] = scale * dl_dforces[i];
     HIPCHK(c, hipMemcpyAsync(c->d_dvbar, dvb.data(), sizeof(double) * dvb.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    const long SF = c->stateFloats();
-    fill_zero_f32(c, c->d_adj, SF);
+    fill_zero_f32(c, c->d_adj, c->stateFloats());
     fill_zero_f64(c, c->d_thbar, (long)c->B * 4);
-    for (int cp = (S - 1) / c->K; cp >= 0; cp--) {
-        int s0 = cp * c->K;
-        int s1 = std::min(s0 + c->K, S);
-        HIPCHK(c, hipMemcpyAsync(c->d_seg, c->d_ckpt + (size_t)cp * SF, sizeof(float) * SF, hipMemcpyDeviceToDevice,
-                                 c->stream));
-        for (int s = s0; s < s1 - 1; s++) {
-            int k = s - s0;
-            step_ref(c, c->d_seg + (size_t)k * SF, c->d_seg + (size_t)(k + 1) * SF, s, false);
-        }
-        for (int s = s1 - 1; s >= s0; s--) step_adj_ref(c, c->d_seg + (size_t)(s - s0) * SF, s);
-    }
+    if (c->mode == DS_KERNELS_TILED) RCHK(backward_tiled(c));
+    else RCHK(backward_ref(c));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipMemcpyAsync(dl_dparams, c->d_thbar, sizeof(double) * 4 * c->B, hipMemcpyDeviceToHost, c->stream));
@@ -452,40 +777,25 @@ int ds_backward(ds_ctx* c, const double* dl_dforces, double* dl_dparams)
     return check_flag(c, "ds_backward");
 }
 
-static int read_group(ds_ctx* c, int gidx, int slot0, int ncomp, float* out)
-{
-    if (!out) return DS_OK;
-    const Group& g = c->groups[gidx];
-    if (g.count == 0) return DS_OK;
-    std::vector<float> tmp((size_t)ncomp * g.count);
-    for (int a = 0; a < ncomp; a++)
-        HIPCHK(c, hipMemcpyAsync(tmp.data() + (size_t)a * g.count, c->d_state + (size_t)(slot0 + a) * c->P + g.offset,
-                                 sizeof(float) * g.count, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int i = 0; i < g.count; i++)
-        for (int a = 0; a < ncomp; a++) out[(size_t)i * ncomp + a] = tmp[(size_t)a * g.count + i];
-    return DS_OK;
-}
-
 int ds_read_liquid(ds_ctx* c, int32_t inst, float* x, float* v, float* C, float* F)
 {
     if (!c) return DS_EINVAL;
     if (inst < 0 || inst >= c->B) return fail(c, DS_EINVAL, "instance index out of range");
-    int g = inst * 3 + LIQUID, rc;
-    if ((rc = read_group(c, g, SX, 3, x))) return rc;
-    if ((rc = read_group(c, g, SV, 3, v))) return rc;
-    if ((rc = read_group(c, g, SC, 9, C))) return rc;
-    return read_group(c, g, SF, 9, F);
+    if (!c->reset_done) return fail(c, DS_ESTATE, "no state: call ds_reset first");
+    RCHK(read_species(c, inst, LIQUID, SX, 3, x));
+    RCHK(read_species(c, inst, LIQUID, SV, 3, v));
+    RCHK(read_species(c, inst, LIQUID, SC, 9, C));
+    return read_species(c, inst, LIQUID, SF, 9, F);
 }
 
 int ds_read_stick(ds_ctx* c, int32_t inst, float* x, float* v, float* C)
 {
     if (!c) return DS_EINVAL;
     if (inst < 0 || inst >= c->B) return fail(c, DS_EINVAL, "instance index out of range");
-    int g = inst * 3 + STICK, rc;
-    if ((rc = read_group(c, g, SX, 3, x))) return rc;
-    if ((rc = read_group(c, g, SV, 3, v))) return rc;
-    return read_group(c, g, SC, 9, C);
+    if (!c->reset_done) return fail(c, DS_ESTATE, "no state: call ds_reset first");
+    RCHK(read_species(c, inst, STICK, SX, 3, x));
+    RCHK(read_species(c, inst, STICK, SV, 3, v));
+    return read_species(c, inst, STICK, SC, 9, C);
 }
 
 int ds_synchronize(ds_ctx* c)
@@ -506,20 +816,22 @@ int ds_last_timing(const ds_ctx* c, double* fwd_ms, double* bwd_ms)
 int ds_svd3(const double* F, int32_t n, double* U, double* S, double* V)
 {
     if (!F || !U || !S || !V || n < 1) return fail(nullptr, DS_EINVAL, "bad argument");
-    double *dF, *dU, *dS, *dV;
-    if (hipMalloc((void**)&dF, 9 * sizeof(double) * n) != hipSuccess) return fail(nullptr, DS_ENOMEM, "hipMalloc");
-    if (hipMalloc((void**)&dU, 9 * sizeof(double) * n) != hipSuccess) { (void)hipFree(dF); return fail(nullptr, DS_ENOMEM, "hipMalloc"); }
-    if (hipMalloc((void**)&dS, 3 * sizeof(double) * n) != hipSuccess) { (void)hipFree(dF); (void)hipFree(dU); return fail(nullptr, DS_ENOMEM, "hipMalloc"); }
-    if (hipMalloc((void**)&dV, 9 * sizeof(double) * n) != hipSuccess) { (void)hipFree(dF); (void)hipFree(dU); (void)hipFree(dS); return fail(nullptr, DS_ENOMEM, "hipMalloc"); }
-    hipError_t e = hipMemcpy(dF, F, 9 * sizeof(double) * n, hipMemcpyHostToDevice);
+    const size_t nb9 = 9 * sizeof(double) * n, nb3 = 3 * sizeof(double) * n;
+    double* buf;
+    if (hipMalloc((void**)&buf, 3 * nb9 + nb3) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(nullptr, DS_ENOMEM, "hipMalloc");
+    }
+    double *dF = buf, *dU = dF + 9 * (size_t)n, *dV = dU + 9 * (size_t)n, *dS = dV + 9 * (size_t)n;
+    hipError_t e = hipMemcpy(dF, F, nb9, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_svd3_test, dim3(nblocks(n)), dim3(256), 0, 0, dF, n, dU, dS, dV);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpy(U, dU, 9 * sizeof(double) * n, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(S, dS, 3 * sizeof(double) * n, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(V, dV, 9 * sizeof(double) * n, hipMemcpyDeviceToHost);
-    (void)hipFree(dF); (void)hipFree(dU); (void)hipFree(dS); (void)hipFree(dV);
+    if (e == hipSuccess) e = hipMemcpy(U, dU, nb9, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(S, dS, nb3, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(V, dV, nb9, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
     if (e != hipSuccess) return fail(nullptr, DS_EHIP, hipGetErrorString(e));
     return DS_OK;
 }
@@ -530,7 +842,10 @@ int ds_svd3_backward(const double* U, const double* S, const double* V, const do
     if (!U || !S || !V || !gU || !gS || !gV || !gF || n < 1) return fail(nullptr, DS_EINVAL, "bad argument");
     const size_t nb9 = 9 * sizeof(double) * n, nb3 = 3 * sizeof(double) * n;
     double* buf;
-    if (hipMalloc((void**)&buf, 5 * nb9 + 2 * nb3) != hipSuccess) return fail(nullptr, DS_ENOMEM, "hipMalloc");
+    if (hipMalloc((void**)&buf, 5 * nb9 + 2 * nb3) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(nullptr, DS_ENOMEM, "hipMalloc");
+    }
     double *dU = buf, *dV = dU + 9 * (size_t)n, *dgU = dV + 9 * (size_t)n, *dgV = dgU + 9 * (size_t)n,
            *dgF = dgV + 9 * (size_t)n, *dS = dgF + 9 * (size_t)n, *dgS = dS + 3 * (size_t)n;
     hipError_t e = hipMemcpy(dU, U, nb9, hipMemcpyHostToDevice);

@@ -1,0 +1,636 @@
+// kernels_tiled.hip -- the substep and its adjoint on sorted particles with LDS grid tiles.
+//
+// Particles are re-sorted at every checkpoint (every K substeps) into buckets of
+// (instance, species, 4x4x4-cell block).  Inside a bucket the order is round-robin over the
+// block's cells (the first particle of every occupied cell, then the second, ...), so the 64
+// lanes of a wavefront sit in different cells and their LDS atomics hit different nodes.  A
+// bucket is cut into chunks of up to 256 particles; one workgroup takes one chunk at a time.
+//
+// Each chunk works on an LDS tile of 8x8x8 grid nodes covering [o-1, o+7) per axis (o = the
+// block's first cell), i.e. every stencil whose base cell lies in [o-1, o+4]: particles may
+// drift one cell out of their block within a segment.  A particle that drifts further uses the
+// global-memory path for its nodes (same arithmetic, slower).
+//
+//   k_p2g_tiled       P2G into the LDS tile, then one atomic flush per touched node
+//   k_g2p_tiled       grid update fused into the tile load (no separate grid pass), gather,
+//                     particle update, stick force; also clears the next step's grid
+//   k_g2p_adj_tiled   adjoint G2P: gathers v_grid, scatters dL/dv_grid through an LDS tile
+//   k_p2g_adj_tiled   adjoint grid update fused into the tile load, adjoint P2G gather,
+//                     FP64 parameter-gradient reduction; clears the next dL/dv_grid buffer
+// LDS tiles are component planes with strides (x 68, y 8, z 1): for the 64 cells of a block
+// the node offsets of one stencil point fall in 32 distinct banks per half-wavefront.
+#include "ds_particle.h"
+
+namespace ds {
+
+constexpr int TB = 4;                 // cells per block axis
+constexpr int TS = 8;                 // tile nodes per axis
+constexpr int TSY = 8;
+constexpr int TSX = 68;
+constexpr int TILE = TS * TSX;        // floats per component plane
+constexpr int TNODES = TS * TS * TS;
+constexpr int WG = 256;               // threads per workgroup = max particles per chunk
+
+struct Chunk {
+    int inst, species, ox, oy, oz, start, count, pad;
+};
+
+__device__ __forceinline__ int tile_id(int lx, int ly, int lz) { return lx * TSX + ly * TSY + lz; }
+
+__device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0, int z0)
+{
+    const unsigned a = (unsigned)(s.b[0] - x0), b = (unsigned)(s.b[1] - y0), c = (unsigned)(s.b[2] - z0);
+    return a <= (unsigned)(TS - 3) && b <= (unsigned)(TS - 3) && c <= (unsigned)(TS - 3);
+}
+
+struct TileScatter4 {
+    float* t;
+    int x0, y0, z0;
+    bool in_tile;
+    ScatterGlobal4 fb;
+    __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c, float d)
+    {
+        if (in_tile) {
+            const int id = tile_id(i - x0, j - y0, k - z0);
+            atomicAdd(&t[id], a);
+            atomicAdd(&t[TILE + id], b);
+            atomicAdd(&t[2 * TILE + id], c);
+            atomicAdd(&t[3 * TILE + id], d);
+        } else {
+            fb.add(i, j, k, a, b, c, d);
+        }
+    }
+};
+
+struct TileScatter3 {
+    float* t;
+    int x0, y0, z0;
+    bool in_tile;
+    ScatterGlobal3 fb;
+    __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c)
+    {
+        if (in_tile) {
+            const int id = tile_id(i - x0, j - y0, k - z0);
+            atomicAdd(&t[id], a);
+            atomicAdd(&t[TILE + id], b);
+            atomicAdd(&t[2 * TILE + id], c);
+        } else {
+            fb.add(i, j, k, a, b, c);
+        }
+    }
+};
+
+struct TileGatherV {
+    const float* t;
+    int x0, y0, z0;
+    bool in_tile;
+    GatherVfromP fb;
+    __device__ __forceinline__ float3 get(int i, int j, int k) const
+    {
+        if (in_tile) {
+            const int id = tile_id(i - x0, j - y0, k - z0);
+            return make_float3(t[id], t[TILE + id], t[2 * TILE + id]);
+        }
+        return fb.get(i, j, k);
+    }
+};
+
+struct TileGatherPbar {
+    const float* t;
+    int x0, y0, z0;
+    bool in_tile;
+    GatherPbarFromP fb;
+    __device__ __forceinline__ float4 get(int i, int j, int k) const
+    {
+        if (in_tile) {
+            const int id = tile_id(i - x0, j - y0, k - z0);
+            return make_float4(t[id], t[TILE + id], t[2 * TILE + id], t[3 * TILE + id]);
+        }
+        return fb.get(i, j, k);
+    }
+};
+
+__device__ __forceinline__ void load_particle_t(const float* __restrict__ st, long LP, int p, bool with_F, ParticleIn& q)
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) q.x[a] = st[(SX + a) * LP + p];
+#pragma unroll
+    for (int a = 0; a < 3; a++) q.v[a] = st[(SV + a) * LP + p];
+#pragma unroll
+    for (int a = 0; a < 9; a++) q.C[a] = st[(SC + a) * LP + p];
+#pragma unroll
+    for (int a = 0; a < 9; a++) q.F[a] = with_F ? st[(SF + a) * LP + p] : ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f);
+}
+
+// Grid-stride clear of a float4 buffer, spread over the workgroups of a chunk kernel.
+__device__ __forceinline__ void clear_slice(float4* __restrict__ buf, long n)
+{
+    if (!buf) return;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
+        buf[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------- forward
+
+__global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
+                                                  const Chunk* __restrict__ chunks, int nchunks,
+                                                  const double* __restrict__ params, float4* __restrict__ grid,
+                                                  DevScene sc, int P, unsigned* __restrict__ flag)
+{
+    __shared__ float tl[4 * TILE];
+    const int n = sc.n_grid;
+    const long nn = (long)n * n * n;
+    const long LP = P;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        for (int e = threadIdx.x; e < 4 * TILE; e += WG) tl[e] = 0.0f;
+        __syncthreads();
+        const Chunk ch = chunks[c];
+        const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
+        float4* gi = grid + (long)ch.inst * nn;
+        if ((int)threadIdx.x < ch.count) {
+            const int p = ch.start + threadIdx.x;
+            ParticleIn q;
+            load_particle_t(in, LP, p, ch.species == LIQUID, q);
+            Stencil s;
+            if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
+            const InstParams ip = load_params(sc, params, ch.inst);
+            TileScatter4 acc{tl, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), ScatterGlobal4{gi, n}};
+            float Fn[9];
+            p2g_particle(sc, ch.species, ip, q, s, Fn, acc);
+            if (fout && ch.species == LIQUID) {
+#pragma unroll
+                for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < TNODES; e += WG) {
+            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            if (gx < 0 || gy < 0 || gz < 0 || gx >= n || gy >= n || gz >= n) continue;
+            const int id = tile_id(lx, ly, lz);
+            const float a = tl[id], b = tl[TILE + id], cc = tl[2 * TILE + id], d = tl[3 * TILE + id];
+            if (a != 0.0f || b != 0.0f || cc != 0.0f || d != 0.0f) {
+                float4* q = gi + node_index(n, gx, gy, gz);
+                atomicAdd(&q->x, a);
+                atomicAdd(&q->y, b);
+                atomicAdd(&q->z, cc);
+                atomicAdd(&q->w, d);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Velocity tile: the grid update applied to the momentum/mass nodes of the tile.
+__device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __restrict__ gi, const DevScene& sc,
+                                                   int x0, int y0, int z0)
+{
+    const int n = sc.n_grid;
+    for (int e = threadIdx.x; e < TNODES; e += WG) {
+        const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) {
+            NodeRec r;
+            gridop_node(sc, gi[node_index(n, gx, gy, gz)], gx, gy, gz, v, r);
+        }
+        const int id = tile_id(lx, ly, lz);
+        tv[id] = v[0];
+        tv[TILE + id] = v[1];
+        tv[2 * TILE + id] = v[2];
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
+                                                  const Chunk* __restrict__ chunks, int nchunks,
+                                                  const float4* __restrict__ grid, float4* __restrict__ clear_next,
+                                                  long clear_n, const float* __restrict__ traj, int T, int f,
+                                                  double* __restrict__ dvsum, DevScene sc, int P,
+                                                  unsigned* __restrict__ flag)
+{
+    __shared__ float tv[3 * TILE];
+    __shared__ double red[WG / 64][3];
+    clear_slice(clear_next, clear_n);
+    const int n = sc.n_grid;
+    const long nn = (long)n * n * n;
+    const long LP = P;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const Chunk ch = chunks[c];
+        const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
+        const float4* gi = grid + (long)ch.inst * nn;
+        load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        __syncthreads();
+        double dv[3] = {0.0, 0.0, 0.0};
+        if ((int)threadIdx.x < ch.count) {
+            const int p = ch.start + threadIdx.x;
+            float x[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) x[a] = in[(SX + a) * LP + p];
+            Stencil s;
+            if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
+            TileGatherV acc{tv, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherVfromP{gi, sc}};
+            float nv[3], nC[9];
+            g2p_gather(sc, s, acc, nv, nC);
+            const float* tr = traj + ((long)ch.inst * T + f) * 3;
+            if (ch.species == LIQUID) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    out[(SV + a) * LP + p] = nv[a];
+                    out[(SX + a) * LP + p] = x[a] + sc.dt * nv[a];
+                }
+            } else if (ch.species == STICK) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const float vk = (tr[3 + a] - tr[a]) / sc.dt;
+                    dv[a] = (double)(vk - nv[a]);
+                    out[(SV + a) * LP + p] = vk;
+                    out[(SX + a) * LP + p] = x[a] + vk * sc.dt;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    out[(SV + a) * LP + p] = 0.0f;
+                    out[(SX + a) * LP + p] = x[a];
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 9; a++) out[(SC + a) * LP + p] = nC[a];
+        }
+        if (ch.species == STICK && dvsum) {
+            const double s0 = wave_sum(dv[0]), s1 = wave_sum(dv[1]), s2 = wave_sum(dv[2]);
+            const int w = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) {
+                red[w][0] = s0;
+                red[w][1] = s1;
+                red[w][2] = s2;
+            }
+            __syncthreads();
+            if (threadIdx.x < 3) {
+                double acc2 = 0.0;
+                for (int k = 0; k < WG / 64; k++) acc2 += red[k][threadIdx.x];
+                atomicAdd(dvsum + ((long)ch.inst * T + f + 1) * 3 + threadIdx.x, acc2);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------- adjoint
+
+__global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ st, const float* __restrict__ adj,
+                                                      const Chunk* __restrict__ chunks, int nchunks,
+                                                      const float4* __restrict__ grid,
+                                                      const double* __restrict__ dvbar, int T, int f,
+                                                      float4* __restrict__ vbar, float* __restrict__ xbar,
+                                                      DevScene sc, int P)
+{
+    __shared__ float tv[3 * TILE];
+    __shared__ float tb[3 * TILE];
+    const int n = sc.n_grid;
+    const long nn = (long)n * n * n;
+    const long LP = P;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const Chunk ch = chunks[c];
+        const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
+        const float4* gi = grid + (long)ch.inst * nn;
+        float4* bi = vbar + (long)ch.inst * nn;
+        if (ch.species == LIQUID) load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        for (int e = threadIdx.x; e < 3 * TILE; e += WG) tb[e] = 0.0f;
+        __syncthreads();
+        if ((int)threadIdx.x < ch.count) {
+            const int p = ch.start + threadIdx.x;
+            float x[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
+            Stencil s;
+            stencil(sc, x[0], x[1], x[2], s);
+            const bool it = stencil_in_tile(s, x0, y0, z0);
+            float Cb[9];
+#pragma unroll
+            for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
+            TileGatherV gv{tv, x0, y0, z0, it, GatherVfromP{gi, sc}};
+            TileScatter3 acc{tb, x0, y0, z0, it, ScatterGlobal3{bi, n}};
+            float vt[3] = {0.0f, 0.0f, 0.0f}, xb[3];
+            if (ch.species == LIQUID) {
+                float xa[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    xa[a] = adj[(SX + a) * LP + p];
+                    vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
+                }
+                g2p_adj_particle<true>(sc, s, vt, Cb, gv, acc, xb);
+#pragma unroll
+                for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
+            } else {
+                if (ch.species == STICK) {
+                    const double* d = dvbar + ((long)ch.inst * T + f + 1) * 3;
+#pragma unroll
+                    for (int a = 0; a < 3; a++) vt[a] = -(float)d[a];
+                }
+                g2p_adj_particle<false>(sc, s, vt, Cb, gv, acc, xb);
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < TNODES; e += WG) {
+            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            if (gx < 0 || gy < 0 || gz < 0 || gx >= n || gy >= n || gz >= n) continue;
+            const int id = tile_id(lx, ly, lz);
+            const float a = tb[id], b = tb[TILE + id], cc = tb[2 * TILE + id];
+            if (a != 0.0f || b != 0.0f || cc != 0.0f) {
+                float4* q = bi + node_index(n, gx, gy, gz);
+                atomicAdd(&q->x, a);
+                atomicAdd(&q->y, b);
+                atomicAdd(&q->z, cc);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, float* __restrict__ adj,
+                                                      const float* __restrict__ xbar, const Chunk* __restrict__ chunks,
+                                                      int nchunks, const double* __restrict__ params,
+                                                      const float4* __restrict__ grid, const float4* __restrict__ vbar,
+                                                      float4* __restrict__ clear_next, long clear_n,
+                                                      double* __restrict__ thbar, DevScene sc, int P)
+{
+    __shared__ float tq[4 * TILE];
+    __shared__ double red[WG / 64][4];
+    clear_slice(clear_next, clear_n);
+    const int n = sc.n_grid;
+    const long nn = (long)n * n * n;
+    const long LP = P;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const Chunk ch = chunks[c];
+        const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
+        const float4* gi = grid + (long)ch.inst * nn;
+        const float4* bi = vbar + (long)ch.inst * nn;
+        for (int e = threadIdx.x; e < TNODES; e += WG) {
+            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            float4 pb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) {
+                const long id = node_index(n, gx, gy, gz);
+                float v[3];
+                NodeRec r;
+                gridop_node(sc, gi[id], gx, gy, gz, v, r);
+                const float4 b = bi[id];
+                pb = gridop_node_adj(r, b.x, b.y, b.z);
+            }
+            const int id = tile_id(lx, ly, lz);
+            tq[id] = pb.x;
+            tq[TILE + id] = pb.y;
+            tq[2 * TILE + id] = pb.z;
+            tq[3 * TILE + id] = pb.w;
+        }
+        __syncthreads();
+        double tb[4] = {0.0, 0.0, 0.0, 0.0};
+        if ((int)threadIdx.x < ch.count) {
+            const int p = ch.start + threadIdx.x;
+            if (ch.species == LIQUID) {
+                ParticleIn q;
+                load_particle_t(st, LP, p, true, q);
+                Stencil s;
+                stencil(sc, q.x[0], q.x[1], q.x[2], s);
+                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
+                const InstParams ip = load_params(sc, params, ch.inst);
+                float Fnb[9], xp[3], xo[3], vo[3], Co[9], Fo[9];
+#pragma unroll
+                for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
+#pragma unroll
+                for (int a = 0; a < 3; a++) xp[a] = xbar[a * LP + p];
+                p2g_adj_liquid(sc, ip, q, s, Fnb, xp, acc, xo, vo, Co, Fo, tb);
+#pragma unroll
+                for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
+#pragma unroll
+                for (int a = 0; a < 3; a++) adj[(SV + a) * LP + p] = vo[a];
+#pragma unroll
+                for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
+#pragma unroll
+                for (int a = 0; a < 9; a++) adj[(SF + a) * LP + p] = Fo[a];
+            } else {
+                float x[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
+                Stencil s;
+                stencil(sc, x[0], x[1], x[2], s);
+                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
+                float Co[9];
+                p2g_adj_rigid(sc, species_mass(sc, ch.species), s, acc, Co);
+#pragma unroll
+                for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
+            }
+        }
+        if (ch.species == LIQUID) {
+            const double r0 = wave_sum(tb[0]), r1 = wave_sum(tb[1]), r2 = wave_sum(tb[2]), r3 = wave_sum(tb[3]);
+            const int w = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) {
+                red[w][0] = r0;
+                red[w][1] = r1;
+                red[w][2] = r2;
+                red[w][3] = r3;
+            }
+            __syncthreads();
+            if (threadIdx.x < 4) {
+                double a2 = 0.0;
+                for (int k = 0; k < WG / 64; k++) a2 += red[k][threadIdx.x];
+                atomicAdd(thbar + 4 * ch.inst + threadIdx.x, a2);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------- bucket sort
+
+// Bucket of a particle: (instance, species, block) with the block from its stencil base cell.
+__global__ void __launch_bounds__(256) k_sort_count(const float* __restrict__ st, const int* __restrict__ meta,
+                                                    const Group* __restrict__ groups, int* __restrict__ key,
+                                                    int* __restrict__ rank, int* __restrict__ cnt, DevScene sc, int NB,
+                                                    int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int g = meta[p];
+    if (g < 0) {
+        key[p] = -1;
+        return;
+    }
+    const Group G = groups[g];
+    const long LP = P;
+    Stencil s;
+    stencil(sc, st[(SX + 0) * LP + p], st[(SX + 1) * LP + p], st[(SX + 2) * LP + p], s);
+    const int bx = s.b[0] / TB, by = s.b[1] / TB, bz = s.b[2] / TB;
+    const int cib = (s.b[0] % TB) * TB * TB + (s.b[1] % TB) * TB + (s.b[2] % TB);
+    const int bucket = ((G.inst * 3 + G.species) * NB + bx) * NB * NB + by * NB + bz;
+    const int cell = bucket * 64 + cib;
+    key[p] = cell;
+    rank[p] = atomicAdd(&cnt[cell], 1);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// One wavefront per bucket: particle count, chunk count and each cell's offset in the bucket.
+__global__ void __launch_bounds__(256) k_bucket_totals(const int* __restrict__ cnt, int* __restrict__ cell_off,
+                                                       int* __restrict__ bcount, int* __restrict__ bnch, int NBK,
+                                                       int chunk)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int bucket = (int)(t >> 6);
+    if (bucket >= NBK) return;
+    const int lane = threadIdx.x & 63;
+    const int nc = cnt[(long)bucket * 64 + lane];
+    const int incl = wave_incl_scan(nc);
+    cell_off[(long)bucket * 64 + lane] = incl - nc;
+    const int total = __shfl(incl, 63, 64);
+    if (lane == 0) {
+        bcount[bucket] = total;
+        bnch[bucket] = (total + chunk - 1) / chunk;
+    }
+}
+
+// Exclusive scans of two int arrays by one workgroup of 1024 threads; tot[0], tot[1] get the sums.
+__global__ void __launch_bounds__(1024) k_scan2(const int* __restrict__ a, const int* __restrict__ b,
+                                                int* __restrict__ oa, int* __restrict__ ob, int n,
+                                                int* __restrict__ tot)
+{
+    __shared__ int wa[16], wb[16];
+    __shared__ int carry[2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int va = i < n ? a[i] : 0, vb = i < n ? b[i] : 0;
+        const int sa = wave_incl_scan(va), sb = wave_incl_scan(vb);
+        if (lane == 63) {
+            wa[w] = sa;
+            wb[w] = sb;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int ra = carry[0], rb = carry[1];
+            for (int k = 0; k < 16; k++) {
+                const int ta = wa[k], tb2 = wb[k];
+                wa[k] = ra;
+                wb[k] = rb;
+                ra += ta;
+                rb += tb2;
+            }
+            carry[0] = ra;
+            carry[1] = rb;
+        }
+        __syncthreads();
+        if (i < n) {
+            oa[i] = wa[w] + sa - va;
+            ob[i] = wb[w] + sb - vb;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tot[0] = carry[0];
+        tot[1] = carry[1];
+    }
+}
+
+// One wavefront per bucket: its chunk entries, and the round-robin slot of every
+// (cell, rank) pair: slot = sum over earlier rounds of their occupied cells + the occupied
+// cells before this one in this round.
+__global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ cnt, const int* __restrict__ cell_off,
+                                                       const int* __restrict__ bstart, const int* __restrict__ cstart,
+                                                       int* __restrict__ rrpos, Chunk* __restrict__ chunks, int NB,
+                                                       int NBK, int chunk)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int bucket = (int)(t >> 6);
+    if (bucket >= NBK) return;
+    const int lane = threadIdx.x & 63;
+    const int nc = cnt[(long)bucket * 64 + lane];
+    if (__ballot(nc > 0) == 0ull) return;
+    int total = nc, mx = nc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        total += __shfl_xor(total, o, 64);
+        mx = max(mx, __shfl_xor(mx, o, 64));
+    }
+    const int bs = bstart[bucket];
+    const int nb3 = NB * NB * NB;
+    const int inst = bucket / (3 * nb3), sp = (bucket / nb3) % 3, blk = bucket % nb3;
+    const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
+    const int nch = (total + chunk - 1) / chunk;
+    for (int i = lane; i < nch; i += 64) {
+        Chunk c;
+        c.inst = inst;
+        c.species = sp;
+        c.ox = bx * TB;
+        c.oy = by * TB;
+        c.oz = bz * TB;
+        c.start = bs + i * chunk;
+        c.count = min(chunk, total - i * chunk);
+        c.pad = 0;
+        chunks[cstart[bucket] + i] = c;
+    }
+    const int off = cell_off[(long)bucket * 64 + lane];
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int S = 0;
+    for (int r = 0; r < mx; r++) {
+        const bool act = nc > r;
+        const unsigned long long m = __ballot(act);
+        if (act) rrpos[bs + off + r] = S + __popcll(m & below);
+        S += __popcll(m);
+    }
+}
+
+// Move every particle to its sorted slot; src[q] records where slot q came from.
+__global__ void __launch_bounds__(256) k_sort_scatter(const float* __restrict__ in, float* __restrict__ out,
+                                                      const int* __restrict__ key, const int* __restrict__ rank,
+                                                      const int* __restrict__ cell_off, const int* __restrict__ bstart,
+                                                      const int* __restrict__ rrpos, const int* __restrict__ meta_in,
+                                                      int* __restrict__ meta_out, const int* __restrict__ pid_in,
+                                                      int* __restrict__ pid_out, int* __restrict__ src, int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int cell = key[p];
+    if (cell < 0) return;
+    const int bs = bstart[cell / 64];
+    const int q = bs + rrpos[bs + cell_off[cell] + rank[p]];
+    const long LP = P;
+#pragma unroll
+    for (int a = 0; a < NSLOT; a++) out[a * LP + q] = in[a * LP + p];
+    meta_out[q] = meta_in[p];
+    pid_out[q] = pid_in[p];
+    src[q] = p;
+}
+
+__global__ void __launch_bounds__(256) k_fill_i32(int* __restrict__ p, long n, int v)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Adjoint carried across a re-sort: slot q of the later order came from slot src[q] of the
+// earlier order.
+__global__ void __launch_bounds__(256) k_unsort_adj(const float* __restrict__ in, float* __restrict__ out,
+                                                    const int* __restrict__ src, int nreal, int P)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nreal) return;
+    const int p = src[q];
+    const long LP = P;
+#pragma unroll
+    for (int a = 0; a < NSLOT; a++) out[a * LP + p] = in[a * LP + q];
+}
+
+}  // namespace ds

@@ -42,7 +42,10 @@ _SHARED = ("n_grid", "dt", "bound", "gravity", "model", "k_bulk", "k_from_param"
 class Engine:
     """B instances advanced together; forward/backward run every instance."""
 
-    def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None):
+    KERNELS = {"tiled": 0, "per_particle": 1}
+
+    def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None,
+                 kernels: str = "tiled"):
         self.lib = _lib.load()
         scenes = list(scenes)
         if not scenes:
@@ -64,6 +67,10 @@ class Engine:
         h = C.c_void_p()
         _lib.check(self.lib.ds_create(C.byref(self._desc), self.B, nl, ns, nc, self.T, self.K, C.byref(h)), None)
         self.ctx = h
+        if kernels not in self.KERNELS:
+            raise ValueError(f"kernels must be one of {list(self.KERNELS)}")
+        self.kernels = kernels
+        self._check(self.lib.ds_set_option(self.ctx, 1, self.KERNELS[kernels]))
         for b, s in enumerate(scenes):
             xl = _f32(s.x_liquid, (-1, 3))
             xs = _f32(s.x_stick, (-1, 3))

@@ -40,6 +40,13 @@ enum {
 
 enum { DS_MODEL_CARREAU = 0, DS_MODEL_CROSS = 1, DS_MODEL_HERSCHEL_BULKLEY = 2 };
 
+/* Options of ds_set_option (no counterpart in the reference: they choose how the substep is
+ * executed, never what it computes). */
+enum {
+    DS_OPT_KERNELS = 1,   /* DS_KERNELS_TILED (default) or DS_KERNELS_PER_PARTICLE; before ds_reset */
+};
+enum { DS_KERNELS_TILED = 0, DS_KERNELS_PER_PARTICLE = 1 };
+
 /* Scene constants shared by all instances of a context.
  * Reference: initialization.py:14-54 (constants), simulation_functions.py:104-225 (hard-coded
  * k = 20000, wall radius 0.0415 - dx/2, bound 3, g = -9.81). */
@@ -63,6 +70,10 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
               const int32_t* n_stick, const int32_t* n_container, int32_t max_steps,
               int32_t ckpt_every, ds_ctx** out);
 void ds_destroy(ds_ctx* ctx);
+/* Set an execution option (DS_OPT_*).  DS_OPT_KERNELS selects the kernel family: sorted
+ * particles with LDS grid tiles (default) or one thread per particle with global atomics; it
+ * takes effect at the next ds_reset. */
+int ds_set_option(ds_ctx* ctx, int32_t option, int32_t value);
 const char* ds_last_error(const ds_ctx* ctx);
 
 /* Initial particle positions of one instance ([n][3] each; NULL allowed for an empty species).

@@ -28,8 +28,11 @@ def o64():
     return Oracle("f64")
 
 
-def gpu_rollout_grad(sc, nsteps, target, ckpt=10):
-    with Engine([sc], ckpt_every=ckpt, max_steps=nsteps + 1) as e:
+KERNELS = ["tiled", "per_particle"]
+
+
+def gpu_rollout_grad(sc, nsteps, target, ckpt=10, kernels="tiled"):
+    with Engine([sc], ckpt_every=ckpt, max_steps=nsteps + 1, kernels=kernels) as e:
         e.reset()
         e.forward(nsteps)
         F = e.forces()[0]
@@ -38,13 +41,14 @@ def gpu_rollout_grad(sc, nsteps, target, ckpt=10):
     return F, L, dth
 
 
+@pytest.mark.parametrize("kernels", KERNELS)
 @pytest.mark.parametrize("name", ["tiny_hb", "ref_carreau", "ref_cross", "ref_hb", "plain_n32_20"])
-def test_golden_forces_and_gradient(name):
+def test_golden_forces_and_gradient(name, kernels):
     sc, z = gold_scene(name)
     n = int(z["nsteps"])
-    print(f"[gpu] golden {name}: {sc.n_liquid} liquid, {sc.n_stick} stick, {sc.n_container} container, {n} steps",
-          flush=True)
-    F, L, g = gpu_rollout_grad(sc, n, z["target"], ckpt=7)
+    print(f"[gpu] golden {name} ({kernels}): {sc.n_liquid} liquid, {sc.n_stick} stick, {sc.n_container} container, "
+          f"{n} steps", flush=True)
+    F, L, g = gpu_rollout_grad(sc, n, z["target"], ckpt=7, kernels=kernels)
     assert F[0].tolist() == [0.0, 0.0, 0.0]
     fe = force_err(F, z["forces"])
     ge = grad_err(g, z["dl_dparams"])
@@ -54,8 +58,9 @@ def test_golden_forces_and_gradient(name):
     assert ge <= GRAD_TOL
 
 
+@pytest.mark.parametrize("kernels", KERNELS)
 @pytest.mark.parametrize("seed,model", [(0, 2), (1, 0), (2, 1)])
-def test_live_oracle_random_scenes(o64, seed, model):
+def test_live_oracle_random_scenes(o64, seed, model, kernels):
     """Seeded variations checked against the oracle run live (not only the fixtures)."""
     rng = np.random.default_rng(100 + seed)
     if model == 2:
@@ -72,18 +77,19 @@ def test_live_oracle_random_scenes(o64, seed, model):
     tgt = 0.7 * Fo
     L0, g0 = loss_and_grad(Fo, tgt, sc.cfg.loss_kind)
     _, dth_o = o64.rollout_grad(sc, g0, nsteps=30, ckpt=6)
-    F, L, dth = gpu_rollout_grad(sc, 30, tgt, ckpt=6)
+    F, L, dth = gpu_rollout_grad(sc, 30, tgt, ckpt=6, kernels=kernels)
     print(f"  force err {force_err(F, Fo):.3e} grad err {grad_err(dth, dth_o):.3e}")
     assert force_err(F, Fo) <= FORCE_TOL
     assert grad_err(dth, dth_o) <= GRAD_TOL
 
 
-def test_batch_matches_single_instances():
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_batch_matches_single_instances(kernels):
     """Uneven instances advanced together give each instance's own result."""
     scenes = [S.tiny_scene(16, 25, seed=s, fill_height=h, params=S.hb_params_draw(np.random.default_rng(s)))
               for s, h in [(0, 0.15), (1, 0.25), (2, 0.2)]]
-    print("[gpu] batch of 3 vs singles", [s.n_liquid for s in scenes], flush=True)
-    with Engine(scenes, ckpt_every=5) as e:
+    print(f"[gpu] batch of 3 vs singles ({kernels})", [s.n_liquid for s in scenes], flush=True)
+    with Engine(scenes, ckpt_every=5, kernels=kernels) as e:
         e.reset()
         e.forward()
         Fb = e.forces()
@@ -91,22 +97,24 @@ def test_batch_matches_single_instances():
         gs = np.stack([compute_loss(Fb[b], tg[b], 1)[1] for b in range(3)])
         db = e.backward(gs)
     for b, sc in enumerate(scenes):
-        F, L, d = gpu_rollout_grad(sc, 25, tg[b], ckpt=5)
+        F, L, d = gpu_rollout_grad(sc, 25, tg[b], ckpt=5, kernels=kernels)
         assert force_err(Fb[b], F) <= 1e-5
         assert grad_err(db[b], d) <= 1e-5
 
 
-def test_checkpoint_interval_does_not_change_gradient():
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_checkpoint_interval_does_not_change_gradient(kernels):
     sc = S.tiny_scene(16, 24, seed=5)
-    print("[gpu] checkpoint interval invariance", flush=True)
-    F0, _, _ = gpu_rollout_grad(sc, 24, np.zeros((25, 3)), ckpt=24)
+    print(f"[gpu] checkpoint interval invariance ({kernels})", flush=True)
+    F0, _, _ = gpu_rollout_grad(sc, 24, np.zeros((25, 3)), ckpt=24, kernels=kernels)
     tgt = 0.5 * F0
-    grads = [gpu_rollout_grad(sc, 24, tgt, ckpt=k)[2] for k in (1, 5, 24)]
+    grads = [gpu_rollout_grad(sc, 24, tgt, ckpt=k, kernels=kernels)[2] for k in (1, 5, 24)]
     for g in grads[1:]:
         assert grad_err(g, grads[0]) <= 1e-5
 
 
-def test_stick_alone_gives_centripetal_force():
+@pytest.mark.parametrize("kernels", KERNELS)
+def test_stick_alone_gives_centripetal_force(kernels):
     """A stick with no liquid or container moves rigidly: its grid velocity is its own velocity,
     so F(s) = M (v_kin(s-1) - v_kin(s-2)) / dt -> M w^2 R towards the circle's centre, and
     F(1) = M v_kin(0) / dt (the start-up step, v = 0 at step 0)."""
@@ -115,7 +123,7 @@ def test_stick_alone_gives_centripetal_force():
     cfg = dataclasses.replace(sc.cfg, wall_r=10.0)
     sc = dataclasses.replace(sc, cfg=cfg, x_liquid=np.zeros((0, 3), np.float32))
     print(f"[gpu] stick alone: {sc.n_stick} particles", flush=True)
-    with Engine([sc], ckpt_every=10) as e:
+    with Engine([sc], ckpt_every=10, kernels=kernels) as e:
         e.reset()
         e.forward()
         F = e.forces()[0]
@@ -130,6 +138,31 @@ def test_stick_alone_gives_centripetal_force():
     R = 0.15
     mag = np.linalg.norm(F[5:, [0, 2]], axis=1)
     np.testing.assert_allclose(mag, M * w * w * R, rtol=2e-2)
+
+
+def test_tiled_matches_per_particle_kernels_over_a_longer_rollout():
+    """The two kernel families on the 18.5k-particle n_grid 32 scene for 150 substeps with two
+    re-sorts per checkpoint interval of 60: state, forces and gradient agree to FP32
+    accumulation-order noise."""
+    sc = S.workload("plain_n32")[0].truncated(151)
+    print(f"[gpu] tiled vs per-particle: {sc.n_liquid} particles, 150 steps", flush=True)
+    out = {}
+    for kernels in KERNELS:
+        with Engine([sc], ckpt_every=60, kernels=kernels) as e:
+            e.reset()
+            e.forward()
+            F = e.forces()[0]
+            st = e.read_liquid(0)
+            stick = e.read_stick(0)
+            _, g = compute_loss(F, 0.5 * F, 1)
+            d = e.backward(g[None])[0]
+        out[kernels] = (F, st, stick, d)
+    (Ft, st_t, sk_t, dt_), (Fp, st_p, sk_p, dp) = out["tiled"], out["per_particle"]
+    assert force_err(Ft, Fp) <= 1e-5
+    for k in ("x", "v", "C", "F"):
+        np.testing.assert_allclose(st_t[k], st_p[k], rtol=1e-4, atol=1e-5 * np.abs(st_p[k]).max())
+    np.testing.assert_allclose(sk_t["x"], sk_p["x"], atol=1e-6)
+    assert grad_err(dt_, dp) <= 1e-4
 
 
 # ------------------------------------------------------------------------- FP64 SVD
