@@ -38,6 +38,8 @@ struct ds_ctx {
     int B = 0, T = 0, K = 0, ncp = 0;
     int P = 0, nreal = 0;
     int mode = DS_KERNELS_TILED;       // requested
+    int ppt = 2;                       // particles per thread of the tiled kernels (chunk = 256 * ppt)
+    int ppt_sorted = 2;                // ppt the current chunk tables were built with
     int alloc_mode = -1;               // mode whose buffers are allocated
     std::vector<int> nl, ns, nc;
     std::vector<Group> groups;
@@ -395,12 +397,14 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     fill_i32(c, c->d_cnt, (long)c->NBK * 64, 0);
     hipLaunchKernelGGL(k_sort_count, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_meta,
                        c->d_groups, c->d_key, c->d_rank, c->d_cnt, c->dsc, c->NB, c->P);
+    const int chunk = WG * c->ppt;
+    c->ppt_sorted = c->ppt;
     hipLaunchKernelGGL(k_bucket_totals, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bcount, c->d_bnch, c->NBK, WG);
+                       c->d_celloff, c->d_bcount, c->d_bnch, c->NBK, chunk);
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, c->stream, c->d_bcount, c->d_bnch, c->d_bstart, c->d_cstart,
                        c->NBK, c->d_tot);
     hipLaunchKernelGGL(k_bucket_layout, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bstart, c->d_cstart, c->d_rrpos, chunks, c->NB, c->NBK, WG);
+                       c->d_celloff, c->d_bstart, c->d_cstart, c->d_rrpos, chunks, c->NB, c->NBK, chunk);
     fill_i32(c, c->d_meta2, c->P, -1);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
                        c->d_rank, c->d_celloff, c->d_bstart, c->d_rrpos, c->d_meta, c->d_meta2, c->d_pid, c->d_pid2,
@@ -417,21 +421,28 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     return DS_OK;
 }
 
+#define DS_PPT_DISPATCH(ppt, kern, ...)                                                          \
+    do {                                                                                         \
+        if ((ppt) == 1) hipLaunchKernelGGL(kern<1>, __VA_ARGS__);                                \
+        else if ((ppt) == 2) hipLaunchKernelGGL(kern<2>, __VA_ARGS__);                           \
+        else hipLaunchKernelGGL(kern<4>, __VA_ARGS__);                                           \
+    } while (0)
+
 void p2g_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* grid)
 {
     const int nch = c->nch_h[cp];
     if (nch == 0) return;
-    hipLaunchKernelGGL(k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout,
-                       c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, grid, c->dsc, c->P, c->d_flag);
+    DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout,
+                    c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, grid, c->dsc, c->P, c->d_flag);
 }
 
 void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, const float4* grid, float4* clear_next, int f,
                bool record)
 {
     const int nch = std::max(c->nch_h[cp], 1);
-    hipLaunchKernelGGL(k_g2p_tiled, dim3(nch), dim3(WG), 0, c->stream, in, out, c->d_chunks + (size_t)cp * c->maxch,
-                       c->nch_h[cp], grid, clear_next, c->gridNodes() * c->B, c->d_traj, c->T, f,
-                       record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
+    DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_tiled, dim3(nch), dim3(WG), 0, c->stream, in, out,
+                    c->d_chunks + (size_t)cp * c->maxch, c->nch_h[cp], grid, clear_next, c->gridNodes() * c->B,
+                    c->d_traj, c->T, f, record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
 }
 
 int forward_tiled(ds_ctx* c, int nsteps)
@@ -482,11 +493,11 @@ int backward_tiled(ds_ctx* c)
         if (nch > 0) {
             for (int k = L - 1; k >= 0; k--) {
                 const int s = s0 + k;
-                hipLaunchKernelGGL(k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj, chunks, nch,
-                                   grid_of(k), c->d_dvbar, c->T, s, c->d_vb2[s & 1], c->d_xbar, c->dsc, c->P);
-                hipLaunchKernelGGL(k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj, c->d_xbar,
-                                   chunks, nch, c->d_params, grid_of(k), c->d_vb2[s & 1], c->d_vb2[(s + 1) & 1], G,
-                                   c->d_thbar, c->dsc, c->P);
+                DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
+                                chunks, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_vb2[s & 1], c->d_xbar, c->dsc, c->P);
+                DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
+                                c->d_xbar, chunks, nch, c->d_params, grid_of(k), c->d_vb2[s & 1], c->d_vb2[(s + 1) & 1],
+                                G, c->d_thbar, c->dsc, c->P);
             }
         }
         if (cp > 0) {
@@ -645,6 +656,13 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
             return fail(c, DS_EINVAL, "DS_OPT_KERNELS takes DS_KERNELS_TILED or DS_KERNELS_PER_PARTICLE");
         if (value != c->mode) c->reset_done = false;
         c->mode = value;
+        return DS_OK;
+    }
+    if (option == DS_OPT_PARTICLES_PER_THREAD) {
+        if (value != 1 && value != 2 && value != 4)
+            return fail(c, DS_EINVAL, "DS_OPT_PARTICLES_PER_THREAD takes 1, 2 or 4");
+        if (value != c->ppt) c->reset_done = false;
+        c->ppt = value;
         return DS_OK;
     }
     return fail(c, DS_EINVAL, "unknown option");

@@ -41,30 +41,44 @@ __device__ __forceinline__ float species_mass(const DevScene& sc, int sp)
 
 // ---------------------------------------------------------------------------- forward
 
-// P2G of one particle (simulation_functions.py:104-196).  For liquid, Fn receives F_tmp.
-template <class Scatter>
-__device__ __forceinline__ void p2g_particle(const DevScene& sc, int sp, const InstParams& ip, const ParticleIn& q,
-                                             const Stencil& s, float Fn[9], Scatter& acc)
+// What one particle scatters in P2G (simulation_functions.py:148-163): affine A, momentum m u
+// and mass m.  For liquid, Fn receives F_tmp.
+struct P2GPayload {
+    float A[9];
+    float mu[3];
+    float m;
+};
+
+__device__ __forceinline__ void p2g_prepare(const DevScene& sc, int sp, const InstParams& ip, const ParticleIn& q,
+                                            float Fn[9], P2GPayload& pl)
 {
-    float A[9], u[3];
-    const float m = species_mass(sc, sp);
+    float u[3];
+    pl.m = species_mass(sc, sp);
     if (sp == LIQUID) {
         LiquidP2G L;
         liquid_p2g(sc, ip.th, ip.kb, q.C, q.F, L);
 #pragma unroll
         for (int a = 0; a < 9; a++) {
             Fn[a] = L.Fn[a];
-            A[a] = L.A[a];
+            pl.A[a] = L.A[a];
         }
         u[0] = q.v[0];
         u[1] = q.v[1] + sc.gdt;
         u[2] = q.v[2];
     } else {
 #pragma unroll
-        for (int a = 0; a < 9; a++) A[a] = m * q.C[a];
+        for (int a = 0; a < 9; a++) pl.A[a] = pl.m * q.C[a];
         u[0] = q.v[0]; u[1] = q.v[1]; u[2] = q.v[2];
     }
-    const float mu[3] = {m * u[0], m * u[1], m * u[2]};
+#pragma unroll
+    for (int a = 0; a < 3; a++) pl.mu[a] = pl.m * u[a];
+}
+
+// The 27-node scatter of P2G: w (m u + A dpos), w m with dpos = (offset - fx) dx.
+template <class Scatter>
+__device__ __forceinline__ void p2g_scatter(const DevScene& sc, const Stencil& s, const P2GPayload& pl, Scatter& acc)
+{
+    const float* A = pl.A;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const float d0 = ((float)i - s.fx[0]) * sc.dx;
@@ -77,12 +91,22 @@ __device__ __forceinline__ void p2g_particle(const DevScene& sc, int sp, const I
                 const float d2 = ((float)k - s.fx[2]) * sc.dx;
                 const float W = wij * s.w[2][k];
                 acc.add(s.b[0] + i, s.b[1] + j, s.b[2] + k,
-                        W * (mu[0] + A[0] * d0 + A[1] * d1 + A[2] * d2),
-                        W * (mu[1] + A[3] * d0 + A[4] * d1 + A[5] * d2),
-                        W * (mu[2] + A[6] * d0 + A[7] * d1 + A[8] * d2), W * m);
+                        W * (pl.mu[0] + A[0] * d0 + A[1] * d1 + A[2] * d2),
+                        W * (pl.mu[1] + A[3] * d0 + A[4] * d1 + A[5] * d2),
+                        W * (pl.mu[2] + A[6] * d0 + A[7] * d1 + A[8] * d2), W * pl.m);
             }
         }
     }
+}
+
+// P2G of one particle (simulation_functions.py:104-196).  For liquid, Fn receives F_tmp.
+template <class Scatter>
+__device__ __forceinline__ void p2g_particle(const DevScene& sc, int sp, const InstParams& ip, const ParticleIn& q,
+                                             const Stencil& s, float Fn[9], Scatter& acc)
+{
+    P2GPayload pl;
+    p2g_prepare(sc, sp, ip, q, Fn, pl);
+    p2g_scatter(sc, s, pl, acc);
 }
 
 // G2P gather of one particle (simulation_functions.py:237-244): nv = sum w v, nC = 4/dx sum w v (x)(o - fx).
@@ -120,12 +144,36 @@ __device__ __forceinline__ void g2p_gather(const DevScene& sc, const Stencil& s,
 
 // ---------------------------------------------------------------------------- adjoint
 
-// Adjoint of G2P for one particle.  vt is the adjoint carried by the gathered velocity
-// (liquid: vbar' + dt xbar'; stick: -dvbar; container: 0), Cb the adjoint of the gathered C.
-// Scatters dL/dv_grid; for liquid (with_x) returns the weight-gradient part of dL/dx.
-template <bool with_x, class Gather, class Scatter>
-__device__ __forceinline__ void g2p_adj_particle(const DevScene& sc, const Stencil& s, const float vt[3],
-                                                 const float Cb[9], Gather& gv, Scatter& acc, float xb[3])
+// Adjoint of G2P for one particle, scatter half.  vt is the adjoint carried by the gathered
+// velocity (liquid: vbar' + dt xbar'; stick: -dvbar; container: 0), Cb the adjoint of the
+// gathered C: dL/dv_grid(node) += w (vt + 4/dx Cb (offset - fx)).
+template <class Scatter>
+__device__ __forceinline__ void g2p_adj_scatter(const DevScene& sc, const Stencil& s, const float vt[3],
+                                                const float Cb[9], Scatter& acc)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
+                const float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
+                float e[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    e[a] = vt[a] + sc.c4 * (Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2]);
+                acc.add(s.b[0] + i, s.b[1] + j, s.b[2] + k, W * e[0], W * e[1], W * e[2]);
+            }
+        }
+    }
+}
+
+// Adjoint of G2P for one liquid particle, position half: the weight-gradient part of dL/dx
+// (the weights and offsets of the gather depend on x).
+template <class Gather>
+__device__ __forceinline__ void g2p_adj_xgrad(const DevScene& sc, const Stencil& s, const float vt[3],
+                                              const float Cb[9], Gather& gv, float xb[3])
 {
     float fxb[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -136,27 +184,35 @@ __device__ __forceinline__ void g2p_adj_particle(const DevScene& sc, const Stenc
             for (int k = 0; k < 3; k++) {
                 const float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
                 const float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
-                float cd[3];
+                float e[3];
 #pragma unroll
-                for (int a = 0; a < 3; a++) cd[a] = Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2];
-                const float e0 = vt[0] + sc.c4 * cd[0], e1 = vt[1] + sc.c4 * cd[1], e2 = vt[2] + sc.c4 * cd[2];
-                acc.add(s.b[0] + i, s.b[1] + j, s.b[2] + k, W * e0, W * e1, W * e2);
-                if (with_x) {
-                    const float3 g = gv.get(s.b[0] + i, s.b[1] + j, s.b[2] + k);
-                    const float Wb = g.x * e0 + g.y * e1 + g.z * e2;
-                    const float dW[3] = {s.dw[0][i] * s.w[1][j] * s.w[2][k], s.w[0][i] * s.dw[1][j] * s.w[2][k],
-                                         s.w[0][i] * s.w[1][j] * s.dw[2][k]};
+                for (int a = 0; a < 3; a++)
+                    e[a] = vt[a] + sc.c4 * (Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2]);
+                const float3 g = gv.get(s.b[0] + i, s.b[1] + j, s.b[2] + k);
+                const float Wb = g.x * e[0] + g.y * e[1] + g.z * e[2];
+                const float dW[3] = {s.dw[0][i] * s.w[1][j] * s.w[2][k], s.w[0][i] * s.dw[1][j] * s.w[2][k],
+                                     s.w[0][i] * s.w[1][j] * s.dw[2][k]};
 #pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        const float ctg = Cb[0 * 3 + c] * g.x + Cb[1 * 3 + c] * g.y + Cb[2 * 3 + c] * g.z;
-                        fxb[c] += Wb * dW[c] - sc.c4 * W * ctg;
-                    }
+                for (int c = 0; c < 3; c++) {
+                    const float ctg = Cb[0 * 3 + c] * g.x + Cb[1 * 3 + c] * g.y + Cb[2 * 3 + c] * g.z;
+                    fxb[c] += Wb * dW[c] - sc.c4 * W * ctg;
                 }
             }
         }
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) xb[a] = fxb[a] * sc.inv_dx;
+}
+
+// Adjoint of G2P for one particle: scatter of dL/dv_grid and, for liquid (with_x), the
+// weight-gradient part of dL/dx.
+template <bool with_x, class Gather, class Scatter>
+__device__ __forceinline__ void g2p_adj_particle(const DevScene& sc, const Stencil& s, const float vt[3],
+                                                 const float Cb[9], Gather& gv, Scatter& acc, float xb[3])
+{
+    g2p_adj_scatter(sc, s, vt, Cb, acc);
+    if (with_x) g2p_adj_xgrad(sc, s, vt, Cb, gv, xb);
+    else xb[0] = xb[1] = xb[2] = 0.0f;
 }
 
 // Adjoint of P2G for one liquid particle.  Gathers (dL/dmomentum, dL/dmass); Fnb_in is the

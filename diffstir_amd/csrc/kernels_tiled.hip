@@ -43,43 +43,36 @@ __device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0
     return a <= (unsigned)(TS - 3) && b <= (unsigned)(TS - 3) && c <= (unsigned)(TS - 3);
 }
 
-struct TileScatter4 {
-    float* t;
+typedef __attribute__((address_space(3))) float lds_float;
+
+// LDS tile accessors (tile planes of TILE floats; node (i, j, k) in global grid coordinates).
+struct LdsScatter4 {
+    lds_float* t;
     int x0, y0, z0;
-    bool in_tile;
-    ScatterGlobal4 fb;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c, float d)
     {
-        if (in_tile) {
-            const int id = tile_id(i - x0, j - y0, k - z0);
-            atomicAdd(&t[id], a);
-            atomicAdd(&t[TILE + id], b);
-            atomicAdd(&t[2 * TILE + id], c);
-            atomicAdd(&t[3 * TILE + id], d);
-        } else {
-            fb.add(i, j, k, a, b, c, d);
-        }
+        lds_float* q = t + tile_id(i - x0, j - y0, k - z0);
+        __builtin_amdgcn_ds_faddf(q, a, 0, 0, false);
+        __builtin_amdgcn_ds_faddf(q + TILE, b, 0, 0, false);
+        __builtin_amdgcn_ds_faddf(q + 2 * TILE, c, 0, 0, false);
+        __builtin_amdgcn_ds_faddf(q + 3 * TILE, d, 0, 0, false);
     }
 };
 
-struct TileScatter3 {
-    float* t;
+struct LdsScatter3 {
+    lds_float* t;
     int x0, y0, z0;
-    bool in_tile;
-    ScatterGlobal3 fb;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c)
     {
-        if (in_tile) {
-            const int id = tile_id(i - x0, j - y0, k - z0);
-            atomicAdd(&t[id], a);
-            atomicAdd(&t[TILE + id], b);
-            atomicAdd(&t[2 * TILE + id], c);
-        } else {
-            fb.add(i, j, k, a, b, c);
-        }
+        lds_float* q = t + tile_id(i - x0, j - y0, k - z0);
+        __builtin_amdgcn_ds_faddf(q, a, 0, 0, false);
+        __builtin_amdgcn_ds_faddf(q + TILE, b, 0, 0, false);
+        __builtin_amdgcn_ds_faddf(q + 2 * TILE, c, 0, 0, false);
     }
 };
 
+// Gather accessors for the tile with a per-node fall-back to global memory for stencils that
+// left the tile (one code path: gathers have no atomics to keep apart).
 struct TileGatherV {
     const float* t;
     int x0, y0, z0;
@@ -131,8 +124,32 @@ __device__ __forceinline__ void clear_slice(float4* __restrict__ buf, long n)
         buf[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+__device__ __forceinline__ void zero_tile(float* t, int planes)
+{
+    for (int e = threadIdx.x; e < planes * TILE; e += WG) t[e] = 0.0f;
+}
+
+// Add the nonzero entries of an LDS tile (ncomp planes) into a float4 grid with one global
+// atomic per component: lane -> (node, component), so a wavefront covers 16 consecutive
+// float4 nodes.
+__device__ __forceinline__ void flush_tile(const float* t, int ncomp, float4* __restrict__ gi, int n, int x0, int y0,
+                                           int z0)
+{
+    float* g = reinterpret_cast<float*>(gi);
+    for (int e = threadIdx.x; e < 4 * TNODES; e += WG) {
+        const int node = e >> 2, comp = e & 3;
+        if (comp >= ncomp) continue;
+        const int lx = node / (TS * TS), ly = (node / TS) % TS, lz = node % TS;
+        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        if (gx < 0 || gy < 0 || gz < 0 || gx >= n || gy >= n || gz >= n) continue;
+        const float v = t[comp * TILE + tile_id(lx, ly, lz)];
+        if (v != 0.0f) atomicAdd(g + 4 * node_index(n, gx, gy, gz) + comp, v);
+    }
+}
+
 // ---------------------------------------------------------------------------- forward
 
+template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
                                                   const Chunk* __restrict__ chunks, int nchunks,
                                                   const double* __restrict__ params, float4* __restrict__ grid,
@@ -143,41 +160,38 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
     const long nn = (long)n * n * n;
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        for (int e = threadIdx.x; e < 4 * TILE; e += WG) tl[e] = 0.0f;
+        zero_tile(tl, 4);
         __syncthreads();
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         float4* gi = grid + (long)ch.inst * nn;
-        if ((int)threadIdx.x < ch.count) {
-            const int p = ch.start + threadIdx.x;
+        const InstParams ip = load_params(sc, params, ch.inst);
+#pragma unroll
+        for (int r = 0; r < PPT; r++) {
+            const int idx = r * WG + threadIdx.x;
+            if (idx >= ch.count) break;
+            const int p = ch.start + idx;
             ParticleIn q;
             load_particle_t(in, LP, p, ch.species == LIQUID, q);
             Stencil s;
             if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
-            const InstParams ip = load_params(sc, params, ch.inst);
-            TileScatter4 acc{tl, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), ScatterGlobal4{gi, n}};
             float Fn[9];
-            p2g_particle(sc, ch.species, ip, q, s, Fn, acc);
+            P2GPayload pl;
+            p2g_prepare(sc, ch.species, ip, q, Fn, pl);
+            if (stencil_in_tile(s, x0, y0, z0)) {
+                LdsScatter4 acc{(lds_float*)tl, x0, y0, z0};
+                p2g_scatter(sc, s, pl, acc);
+            } else {
+                ScatterGlobal4 acc{gi, n};
+                p2g_scatter(sc, s, pl, acc);
+            }
             if (fout && ch.species == LIQUID) {
 #pragma unroll
                 for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < TNODES; e += WG) {
-            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
-            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-            if (gx < 0 || gy < 0 || gz < 0 || gx >= n || gy >= n || gz >= n) continue;
-            const int id = tile_id(lx, ly, lz);
-            const float a = tl[id], b = tl[TILE + id], cc = tl[2 * TILE + id], d = tl[3 * TILE + id];
-            if (a != 0.0f || b != 0.0f || cc != 0.0f || d != 0.0f) {
-                float4* q = gi + node_index(n, gx, gy, gz);
-                atomicAdd(&q->x, a);
-                atomicAdd(&q->y, b);
-                atomicAdd(&q->z, cc);
-                atomicAdd(&q->w, d);
-            }
-        }
+        flush_tile(tl, 4, gi, n, x0, y0, z0);
         __syncthreads();
     }
 }
@@ -202,6 +216,7 @@ __device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __re
     }
 }
 
+template <int PPT>
 __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
                                                   const Chunk* __restrict__ chunks, int nchunks,
                                                   const float4* __restrict__ grid, float4* __restrict__ clear_next,
@@ -222,17 +237,20 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
         load_velocity_tile(tv, gi, sc, x0, y0, z0);
         __syncthreads();
         double dv[3] = {0.0, 0.0, 0.0};
-        if ((int)threadIdx.x < ch.count) {
-            const int p = ch.start + threadIdx.x;
+        const float* tr = traj + ((long)ch.inst * T + f) * 3;
+#pragma unroll
+        for (int r = 0; r < PPT; r++) {
+            const int idx = r * WG + threadIdx.x;
+            if (idx >= ch.count) break;
+            const int p = ch.start + idx;
             float x[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) x[a] = in[(SX + a) * LP + p];
             Stencil s;
             if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
-            TileGatherV acc{tv, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherVfromP{gi, sc}};
             float nv[3], nC[9];
+            TileGatherV acc{tv, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherVfromP{gi, sc}};
             g2p_gather(sc, s, acc, nv, nC);
-            const float* tr = traj + ((long)ch.inst * T + f) * 3;
             if (ch.species == LIQUID) {
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
@@ -243,7 +261,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     const float vk = (tr[3 + a] - tr[a]) / sc.dt;
-                    dv[a] = (double)(vk - nv[a]);
+                    dv[a] += (double)(vk - nv[a]);
                     out[(SV + a) * LP + p] = vk;
                     out[(SX + a) * LP + p] = x[a] + vk * sc.dt;
                 }
@@ -278,6 +296,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
 
 // ---------------------------------------------------------------------------- adjoint
 
+template <int PPT>
 __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ st, const float* __restrict__ adj,
                                                       const Chunk* __restrict__ chunks, int nchunks,
                                                       const float4* __restrict__ grid,
@@ -295,11 +314,21 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const float4* gi = grid + (long)ch.inst * nn;
         float4* bi = vbar + (long)ch.inst * nn;
-        if (ch.species == LIQUID) load_velocity_tile(tv, gi, sc, x0, y0, z0);
-        for (int e = threadIdx.x; e < 3 * TILE; e += WG) tb[e] = 0.0f;
+        const bool liq = ch.species == LIQUID;
+        if (liq) load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        zero_tile(tb, 3);
         __syncthreads();
-        if ((int)threadIdx.x < ch.count) {
-            const int p = ch.start + threadIdx.x;
+        float vstick[3] = {0.0f, 0.0f, 0.0f};
+        if (ch.species == STICK) {
+            const double* d = dvbar + ((long)ch.inst * T + f + 1) * 3;
+#pragma unroll
+            for (int a = 0; a < 3; a++) vstick[a] = -(float)d[a];
+        }
+#pragma unroll
+        for (int r = 0; r < PPT; r++) {
+            const int idx = r * WG + threadIdx.x;
+            if (idx >= ch.count) break;
+            const int p = ch.start + idx;
             float x[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
@@ -309,46 +338,61 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
             float Cb[9];
 #pragma unroll
             for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
-            TileGatherV gv{tv, x0, y0, z0, it, GatherVfromP{gi, sc}};
-            TileScatter3 acc{tb, x0, y0, z0, it, ScatterGlobal3{bi, n}};
-            float vt[3] = {0.0f, 0.0f, 0.0f}, xb[3];
-            if (ch.species == LIQUID) {
-                float xa[3];
+            float vt[3] = {vstick[0], vstick[1], vstick[2]}, xa[3] = {0.0f, 0.0f, 0.0f};
+            if (liq) {
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     xa[a] = adj[(SX + a) * LP + p];
                     vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
                 }
-                g2p_adj_particle<true>(sc, s, vt, Cb, gv, acc, xb);
+            }
+            if (it) {
+                LdsScatter3 acc{(lds_float*)tb, x0, y0, z0};
+                g2p_adj_scatter(sc, s, vt, Cb, acc);
+            } else {
+                ScatterGlobal3 acc{bi, n};
+                g2p_adj_scatter(sc, s, vt, Cb, acc);
+            }
+            if (liq) {
+                float xb[3];
+                TileGatherV gv{tv, x0, y0, z0, it, GatherVfromP{gi, sc}};
+                g2p_adj_xgrad(sc, s, vt, Cb, gv, xb);
 #pragma unroll
                 for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
-            } else {
-                if (ch.species == STICK) {
-                    const double* d = dvbar + ((long)ch.inst * T + f + 1) * 3;
-#pragma unroll
-                    for (int a = 0; a < 3; a++) vt[a] = -(float)d[a];
-                }
-                g2p_adj_particle<false>(sc, s, vt, Cb, gv, acc, xb);
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < TNODES; e += WG) {
-            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
-            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-            if (gx < 0 || gy < 0 || gz < 0 || gx >= n || gy >= n || gz >= n) continue;
-            const int id = tile_id(lx, ly, lz);
-            const float a = tb[id], b = tb[TILE + id], cc = tb[2 * TILE + id];
-            if (a != 0.0f || b != 0.0f || cc != 0.0f) {
-                float4* q = bi + node_index(n, gx, gy, gz);
-                atomicAdd(&q->x, a);
-                atomicAdd(&q->y, b);
-                atomicAdd(&q->z, cc);
-            }
-        }
+        flush_tile(tb, 3, bi, n, x0, y0, z0);
         __syncthreads();
     }
 }
 
+// Adjoint-of-P2G input tile: the grid update's adjoint at every node of the tile.
+__device__ __forceinline__ void load_pbar_tile(float* tq, const float4* __restrict__ gi, const float4* __restrict__ bi,
+                                               const DevScene& sc, int x0, int y0, int z0)
+{
+    const int n = sc.n_grid;
+    for (int e = threadIdx.x; e < TNODES; e += WG) {
+        const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        float4 pb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) {
+            const long id = node_index(n, gx, gy, gz);
+            float v[3];
+            NodeRec r;
+            gridop_node(sc, gi[id], gx, gy, gz, v, r);
+            const float4 b = bi[id];
+            pb = gridop_node_adj(r, b.x, b.y, b.z);
+        }
+        const int id = tile_id(lx, ly, lz);
+        tq[id] = pb.x;
+        tq[TILE + id] = pb.y;
+        tq[2 * TILE + id] = pb.z;
+        tq[3 * TILE + id] = pb.w;
+    }
+}
+
+template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, float* __restrict__ adj,
                                                       const float* __restrict__ xbar, const Chunk* __restrict__ chunks,
                                                       int nchunks, const double* __restrict__ params,
@@ -367,40 +411,26 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const float4* gi = grid + (long)ch.inst * nn;
         const float4* bi = vbar + (long)ch.inst * nn;
-        for (int e = threadIdx.x; e < TNODES; e += WG) {
-            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
-            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-            float4 pb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) {
-                const long id = node_index(n, gx, gy, gz);
-                float v[3];
-                NodeRec r;
-                gridop_node(sc, gi[id], gx, gy, gz, v, r);
-                const float4 b = bi[id];
-                pb = gridop_node_adj(r, b.x, b.y, b.z);
-            }
-            const int id = tile_id(lx, ly, lz);
-            tq[id] = pb.x;
-            tq[TILE + id] = pb.y;
-            tq[2 * TILE + id] = pb.z;
-            tq[3 * TILE + id] = pb.w;
-        }
+        load_pbar_tile(tq, gi, bi, sc, x0, y0, z0);
         __syncthreads();
         double tb[4] = {0.0, 0.0, 0.0, 0.0};
-        if ((int)threadIdx.x < ch.count) {
-            const int p = ch.start + threadIdx.x;
+        const InstParams ip = load_params(sc, params, ch.inst);
+#pragma unroll
+        for (int r = 0; r < PPT; r++) {
+            const int idx = r * WG + threadIdx.x;
+            if (idx >= ch.count) break;
+            const int p = ch.start + idx;
             if (ch.species == LIQUID) {
                 ParticleIn q;
                 load_particle_t(st, LP, p, true, q);
                 Stencil s;
                 stencil(sc, q.x[0], q.x[1], q.x[2], s);
-                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
-                const InstParams ip = load_params(sc, params, ch.inst);
                 float Fnb[9], xp[3], xo[3], vo[3], Co[9], Fo[9];
 #pragma unroll
                 for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
 #pragma unroll
                 for (int a = 0; a < 3; a++) xp[a] = xbar[a * LP + p];
+                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
                 p2g_adj_liquid(sc, ip, q, s, Fnb, xp, acc, xo, vo, Co, Fo, tb);
 #pragma unroll
                 for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
@@ -416,8 +446,8 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
                 Stencil s;
                 stencil(sc, x[0], x[1], x[2], s);
-                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
                 float Co[9];
+                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
                 p2g_adj_rigid(sc, species_mass(sc, ch.species), s, acc, Co);
 #pragma unroll
                 for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];

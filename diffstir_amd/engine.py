@@ -45,7 +45,7 @@ class Engine:
     KERNELS = {"tiled": 0, "per_particle": 1}
 
     def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None,
-                 kernels: str = "tiled"):
+                 kernels: str = "tiled", particles_per_thread: int = 2):
         self.lib = _lib.load()
         scenes = list(scenes)
         if not scenes:
@@ -71,6 +71,7 @@ class Engine:
             raise ValueError(f"kernels must be one of {list(self.KERNELS)}")
         self.kernels = kernels
         self._check(self.lib.ds_set_option(self.ctx, 1, self.KERNELS[kernels]))
+        self._check(self.lib.ds_set_option(self.ctx, 2, int(particles_per_thread)))
         for b, s in enumerate(scenes):
             xl = _f32(s.x_liquid, (-1, 3))
             xs = _f32(s.x_stick, (-1, 3))

@@ -44,6 +44,7 @@ enum { DS_MODEL_CARREAU = 0, DS_MODEL_CROSS = 1, DS_MODEL_HERSCHEL_BULKLEY = 2 }
  * executed, never what it computes). */
 enum {
     DS_OPT_KERNELS = 1,   /* DS_KERNELS_TILED (default) or DS_KERNELS_PER_PARTICLE; before ds_reset */
+    DS_OPT_PARTICLES_PER_THREAD = 2,   /* tiled kernels: 1, 2 (default) or 4 particles per thread */
 };
 enum { DS_KERNELS_TILED = 0, DS_KERNELS_PER_PARTICLE = 1 };
 

@@ -140,6 +140,22 @@ def test_stick_alone_gives_centripetal_force(kernels):
     np.testing.assert_allclose(mag, M * w * w * R, rtol=2e-2)
 
 
+@pytest.mark.parametrize("ppt", [1, 2, 4])
+def test_particles_per_thread_variants_agree(ppt):
+    """The tiled kernels with 1, 2 or 4 particles per thread (chunks of 256, 512, 1024) give the
+    oracle's forces and gradient."""
+    sc, z = gold_scene("plain_n32_20")
+    print(f"[gpu] tiled, {ppt} particles per thread", flush=True)
+    with Engine([sc], ckpt_every=7, max_steps=21, particles_per_thread=ppt) as e:
+        e.reset()
+        e.forward()
+        F = e.forces()[0]
+        _, g = compute_loss(F, z["target"], sc.cfg.loss_kind)
+        d = e.backward(g[None])[0]
+    assert force_err(F, z["forces"]) <= FORCE_TOL
+    assert grad_err(d, z["dl_dparams"]) <= GRAD_TOL
+
+
 def test_tiled_matches_per_particle_kernels_over_a_longer_rollout():
     """The two kernel families on the 18.5k-particle n_grid 32 scene for 150 substeps with two
     re-sorts per checkpoint interval of 60: state, forces and gradient agree to FP32
