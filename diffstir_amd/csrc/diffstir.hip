@@ -71,8 +71,12 @@ struct ds_ctx {
     float4* d_vbar = nullptr;
     float4* d_pbar = nullptr;
     // tiled kernels
-    float4* d_g2[2] = {nullptr, nullptr};
-    float4* d_vb2[2] = {nullptr, nullptr};
+    float4* d_vgrid = nullptr;     // grid velocity of the current step
+    float4* d_pbar2 = nullptr;     // dL/d(momentum, mass) of the current adjoint step
+    float4* d_pfb = nullptr;       // fall-back momentum/mass grid (stencils outside their tile)
+    float4* d_vbfb = nullptr;      // fall-back dL/dv grid
+    float4* d_staging = nullptr;   // per-chunk staging slots
+    long staging_cap = 0;          // slots allocated
     float4* d_seg_grid = nullptr;
     float* d_adj2 = nullptr;
     int* d_meta2 = nullptr;
@@ -81,9 +85,9 @@ struct ds_ctx {
     int* d_cnt = nullptr;
     int* d_celloff = nullptr;
     int* d_bcount = nullptr;
-    int* d_bnch = nullptr;
+    int* d_bnch = nullptr;         // ncp x NBK chunks per bucket
     int* d_bstart = nullptr;
-    int* d_cstart = nullptr;
+    int* d_cstart = nullptr;       // ncp x NBK first chunk of each bucket
     int* d_tot = nullptr;
     int* d_key = nullptr;
     int* d_rank = nullptr;
@@ -220,10 +224,12 @@ void free_mode(ds_ctx* c)
     dfree(c->d_vout);
     dfree(c->d_vbar);
     dfree(c->d_pbar);
-    dfree(c->d_g2[0]);
-    dfree(c->d_g2[1]);
-    dfree(c->d_vb2[0]);
-    dfree(c->d_vb2[1]);
+    dfree(c->d_vgrid);
+    dfree(c->d_pbar2);
+    dfree(c->d_pfb);
+    dfree(c->d_vbfb);
+    dfree(c->d_staging);
+    c->staging_cap = 0;
     dfree(c->d_seg_grid);
     dfree(c->d_adj2);
     dfree(c->d_meta2);
@@ -291,10 +297,10 @@ int alloc_mode(ds_ctx* c)
         c->NB = (c->desc.n_grid + TB - 1) / TB;
         c->NBK = c->B * 3 * c->NB * c->NB * c->NB;
         c->maxch = (long)c->P / 64 + std::min<long>(c->P, c->NBK) + 1;
-        MALLOC(c->d_g2[0], G);
-        MALLOC(c->d_g2[1], G);
-        MALLOC(c->d_vb2[0], G);
-        MALLOC(c->d_vb2[1], G);
+        MALLOC(c->d_vgrid, G);
+        MALLOC(c->d_pbar2, G);
+        MALLOC(c->d_pfb, G);
+        MALLOC(c->d_vbfb, G);
         MALLOC(c->d_seg_grid, (size_t)G * c->K);
         MALLOC(c->d_adj2, SF);
         MALLOC(c->d_meta2, c->P);
@@ -303,9 +309,9 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_cnt, (size_t)c->NBK * 64);
         MALLOC(c->d_celloff, (size_t)c->NBK * 64);
         MALLOC(c->d_bcount, c->NBK);
-        MALLOC(c->d_bnch, c->NBK);
+        MALLOC(c->d_bnch, (size_t)c->NBK * c->ncp);
         MALLOC(c->d_bstart, c->NBK);
-        MALLOC(c->d_cstart, c->NBK);
+        MALLOC(c->d_cstart, (size_t)c->NBK * c->ncp);
         MALLOC(c->d_tot, 2);
         MALLOC(c->d_key, c->P);
         MALLOC(c->d_rank, c->P);
@@ -313,6 +319,8 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_chunks, (size_t)c->maxch * c->ncp);
         MALLOC(c->d_src, (size_t)c->P * c->ncp);
         c->nch_h.assign(c->ncp, 0);
+        fill_zero_f4(c, c->d_pfb, G);
+        fill_zero_f4(c, c->d_vbfb, G);
     }
 #undef MALLOC
     c->alloc_mode = c->mode;
@@ -394,17 +402,19 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     float* dst = c->d_ckpt + (size_t)cp * SF;
     Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
     int* src = c->d_src + (size_t)cp * c->P;
+    int* bnch = c->d_bnch + (size_t)cp * c->NBK;
+    int* cstart = c->d_cstart + (size_t)cp * c->NBK;
     fill_i32(c, c->d_cnt, (long)c->NBK * 64, 0);
     hipLaunchKernelGGL(k_sort_count, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_meta,
                        c->d_groups, c->d_key, c->d_rank, c->d_cnt, c->dsc, c->NB, c->P);
     const int chunk = WG * c->ppt;
     c->ppt_sorted = c->ppt;
     hipLaunchKernelGGL(k_bucket_totals, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bcount, c->d_bnch, c->NBK, chunk);
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, c->stream, c->d_bcount, c->d_bnch, c->d_bstart, c->d_cstart,
-                       c->NBK, c->d_tot);
+                       c->d_celloff, c->d_bcount, bnch, c->NBK, chunk);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, c->stream, c->d_bcount, bnch, c->d_bstart, cstart, c->NBK,
+                       c->d_tot);
     hipLaunchKernelGGL(k_bucket_layout, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bstart, c->d_cstart, c->d_rrpos, chunks, c->NB, c->NBK, chunk);
+                       c->d_celloff, c->d_bstart, cstart, c->d_rrpos, chunks, c->NB, c->NBK, chunk);
     fill_i32(c, c->d_meta2, c->P, -1);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
                        c->d_rank, c->d_celloff, c->d_bstart, c->d_rrpos, c->d_meta, c->d_meta2, c->d_pid, c->d_pid2,
@@ -418,6 +428,13 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
         return fail(c, DS_EHIP, "bucket sort lost particles (" + std::to_string(tot[0]) + " of " +
                                     std::to_string(c->nreal) + ")");
     c->nch_h[cp] = tot[1];
+    if (tot[1] > c->staging_cap) {
+        const long cap = (long)tot[1] + tot[1] / 4 + 16;
+        dfree(c->d_staging);
+        c->staging_cap = 0;
+        RCHK(dalloc(c, &c->d_staging, (size_t)cap * TNODES));
+        c->staging_cap = cap;
+    }
     return DS_OK;
 }
 
@@ -428,27 +445,34 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
         else hipLaunchKernelGGL(kern<4>, __VA_ARGS__);                                           \
     } while (0)
 
-void p2g_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* grid)
+// P2G of a checkpoint segment's chunks into the staging slots, then the grid reduction:
+// velocity into d_vgrid and, when pout is given, momentum/mass into pout.
+void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* pout)
+{
+    const int nch = c->nch_h[cp];
+    const int* bnch = c->d_bnch + (size_t)cp * c->NBK;
+    const int* cstart = c->d_cstart + (size_t)cp * c->NBK;
+    if (nch > 0)
+        DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout,
+                        c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, c->d_staging, c->d_pfb, c->dsc, c->P,
+                        c->d_flag);
+    const int nb3 = c->NB * c->NB * c->NB;
+    hipLaunchKernelGGL(k_grid_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart, c->d_pfb,
+                       pout, c->d_vgrid, c->dsc, c->NB);
+}
+
+void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool record)
 {
     const int nch = c->nch_h[cp];
     if (nch == 0) return;
-    DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout,
-                    c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, grid, c->dsc, c->P, c->d_flag);
-}
-
-void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, const float4* grid, float4* clear_next, int f,
-               bool record)
-{
-    const int nch = std::max(c->nch_h[cp], 1);
     DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_tiled, dim3(nch), dim3(WG), 0, c->stream, in, out,
-                    c->d_chunks + (size_t)cp * c->maxch, c->nch_h[cp], grid, clear_next, c->gridNodes() * c->B,
-                    c->d_traj, c->T, f, record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
+                    c->d_chunks + (size_t)cp * c->maxch, nch, c->d_vgrid, c->d_traj, c->T, f,
+                    record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
 }
 
 int forward_tiled(ds_ctx* c, int nsteps)
 {
     const long SF = c->stateFloats();
-    const long G = c->gridNodes() * c->B;
     for (int i = 0; i < nsteps; i++) {
         const int s = c->step;
         const int cp = s / c->K;
@@ -457,10 +481,8 @@ int forward_tiled(ds_ctx* c, int nsteps)
             RCHK(sort_into_checkpoint(c, cp));
             in = c->d_ckpt + (size_t)cp * SF;
         }
-        float4* g = c->d_g2[s & 1];
-        p2g_tiled(c, in, c->d_state, cp, g);
-        g2p_tiled(c, in, c->d_state, cp, g, c->d_g2[(s + 1) & 1], s, true);
-        (void)G;
+        p2g_reduce_tiled(c, in, c->d_state, cp, nullptr);
+        g2p_tiled(c, in, c->d_state, cp, s, true);
         c->step++;
     }
     return DS_OK;
@@ -471,34 +493,36 @@ int backward_tiled(ds_ctx* c)
     const int S = c->step;
     const long SF = c->stateFloats();
     const long G = c->gridNodes() * c->B;
-    fill_zero_f4(c, c->d_vb2[0], G);
-    fill_zero_f4(c, c->d_vb2[1], G);
+    const int nb3 = c->NB * c->NB * c->NB;
     for (int cp = (S - 1) / c->K; cp >= 0; cp--) {
         const int s0 = cp * c->K;
         const int s1 = std::min(s0 + c->K, S);
         const int L = s1 - s0;
         const int nch = c->nch_h[cp];
         const Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
+        const int* bnch = c->d_bnch + (size_t)cp * c->NBK;
+        const int* cstart = c->d_cstart + (size_t)cp * c->NBK;
         const float* ck = c->d_ckpt + (size_t)cp * SF;
         auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : c->d_seg + (size_t)k * SF; };
         auto grid_of = [&](int k) -> float4* { return c->d_seg_grid + (size_t)k * G; };
         // recompute the segment, keeping each step's state and momentum/mass grid
-        fill_zero_f4(c, grid_of(0), G);
         for (int k = 0; k < L; k++) {
             const int s = s0 + k;
-            p2g_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k));
-            if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, grid_of(k), grid_of(k + 1), s, false);
+            p2g_reduce_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k));
+            if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, s, false);
         }
         // adjoint, last step first
-        if (nch > 0) {
-            for (int k = L - 1; k >= 0; k--) {
-                const int s = s0 + k;
+        for (int k = L - 1; k >= 0; k--) {
+            const int s = s0 + k;
+            if (nch > 0)
                 DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
-                                chunks, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_vb2[s & 1], c->d_xbar, c->dsc, c->P);
+                                chunks, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb, c->d_xbar,
+                                c->dsc, c->P);
+            hipLaunchKernelGGL(k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
+                               c->d_vbfb, grid_of(k), c->d_pbar2, c->dsc, c->NB);
+            if (nch > 0)
                 DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
-                                c->d_xbar, chunks, nch, c->d_params, grid_of(k), c->d_vb2[s & 1], c->d_vb2[(s + 1) & 1],
-                                G, c->d_thbar, c->dsc, c->P);
-            }
+                                c->d_xbar, chunks, nch, c->d_params, c->d_pbar2, c->d_thbar, c->dsc, c->P);
         }
         if (cp > 0) {
             hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
@@ -720,7 +744,6 @@ int ds_reset(ds_ctx* c)
     HIPCHK(c, hipMemcpyAsync(c->d_meta, c->meta_h.data(), sizeof(int) * c->P, hipMemcpyHostToDevice, c->stream));
     if (c->mode == DS_KERNELS_TILED) {
         HIPCHK(c, hipMemcpyAsync(c->d_pid, c->pid_h.data(), sizeof(int) * c->P, hipMemcpyHostToDevice, c->stream));
-        fill_zero_f4(c, c->d_g2[0], c->gridNodes() * c->B);
     }
     hipLaunchKernelGGL(k_init_state, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_x0, c->d_meta,
                        c->P, 0.5f);

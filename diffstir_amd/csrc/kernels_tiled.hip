@@ -1,22 +1,37 @@
-// kernels_tiled.hip -- the substep and its adjoint on sorted particles with LDS grid tiles.
+// kernels_tiled.hip -- the substep and its adjoint on sorted particles with LDS grid tiles,
+// without floating-point atomics on the hot path.
 //
 // Particles are re-sorted at every checkpoint (every K substeps) into buckets of
 // (instance, species, 4x4x4-cell block).  Inside a bucket the order is round-robin over the
 // block's cells (the first particle of every occupied cell, then the second, ...), so the 64
-// lanes of a wavefront sit in different cells and their LDS atomics hit different nodes.  A
-// bucket is cut into chunks of up to 256 particles; one workgroup takes one chunk at a time.
+// lanes of a wavefront sit in different cells.  A bucket is cut into chunks of up to
+// 256 * PPT particles; one workgroup (4 wavefronts, PPT particles per thread) takes a chunk.
 //
-// Each chunk works on an LDS tile of 8x8x8 grid nodes covering [o-1, o+7) per axis (o = the
-// block's first cell), i.e. every stencil whose base cell lies in [o-1, o+4]: particles may
-// drift one cell out of their block within a segment.  A particle that drifts further uses the
-// global-memory path for its nodes (same arithmetic, slower).
+// Each chunk works on an 8x8x8-node tile covering nodes [o-1, o+7) per axis (o = the block's
+// first cell): every stencil whose base cell lies in [o-1, o+4], so particles may drift one
+// cell out of their block within a segment.  A stencil that drifts further goes to a global
+// fall-back grid with atomics (rare).
 //
-//   k_p2g_tiled       P2G into the LDS tile, then one atomic flush per touched node
-//   k_g2p_tiled       grid update fused into the tile load (no separate grid pass), gather,
-//                     particle update, stick force; also clears the next step's grid
-//   k_g2p_adj_tiled   adjoint G2P: gathers v_grid, scatters dL/dv_grid through an LDS tile
-//   k_p2g_adj_tiled   adjoint grid update fused into the tile load, adjoint P2G gather,
-//                     FP64 parameter-gradient reduction; clears the next dL/dv_grid buffer
+// Scatter without atomics.  LDS float atomics (ds_add_f32) run at about 1/24 of the rate of a
+// plain LDS read-modify-write on gfx950, and global float atomics are slow as well.  So:
+//   * every wavefront owns a private LDS tile;
+//   * before scattering, the lanes of a wavefront claim their stencil base node in a per-wave
+//     table; lanes that share a base are served in successive passes, so within one pass the
+//     27 read-modify-writes of different lanes never hit the same node in the same
+//     instruction, and a wavefront's LDS operations complete in program order (volatile
+//     accesses keep the compiler from reordering them);
+//   * the workgroup sums its four wave tiles and stores the sum in the chunk's staging slot
+//     (8 KB of plain coalesced stores);
+//   * a reduction kernel, one thread per grid node, adds the staging slots of the (at most
+//     eight per species) chunks whose tiles cover the node plus the fall-back grid, and applies
+//     the grid update there.  The sum order is fixed, so the grid is deterministic.
+//
+//   k_p2g_tiled       P2G of a chunk into its staging slot
+//   k_grid_reduce     staged momentum/mass -> grid update -> velocity (and momentum/mass)
+//   k_g2p_tiled       gather from a velocity tile, particle update, stick force
+//   k_g2p_adj_tiled   adjoint G2P: velocity tile from momentum/mass, dL/dv scatter to staging
+//   k_vbar_reduce     staged dL/dv -> adjoint grid update -> dL/d(momentum, mass)
+//   k_p2g_adj_tiled   adjoint P2G gather, FP64 parameter-gradient reduction
 // LDS tiles are component planes with strides (x 68, y 8, z 1): for the 64 cells of a block
 // the node offsets of one stencil point fall in 32 distinct banks per half-wavefront.
 #include "ds_particle.h"
@@ -28,8 +43,9 @@ constexpr int TS = 8;                 // tile nodes per axis
 constexpr int TSY = 8;
 constexpr int TSX = 68;
 constexpr int TILE = TS * TSX;        // floats per component plane
-constexpr int TNODES = TS * TS * TS;
-constexpr int WG = 256;               // threads per workgroup = max particles per chunk
+constexpr int TNODES = TS * TS * TS;  // nodes of a staging slot (dense, index lx*64 + ly*8 + lz)
+constexpr int WG = 256;               // threads per workgroup
+constexpr int WAVES = WG / 64;
 
 struct Chunk {
     int inst, species, ox, oy, oz, start, count, pad;
@@ -43,37 +59,52 @@ __device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0
     return a <= (unsigned)(TS - 3) && b <= (unsigned)(TS - 3) && c <= (unsigned)(TS - 3);
 }
 
-typedef __attribute__((address_space(3))) float lds_float;
-
-// LDS tile accessors (tile planes of TILE floats; node (i, j, k) in global grid coordinates).
-struct LdsScatter4 {
-    lds_float* t;
+// Read-modify-write scatter into a wavefront's private tile (see the claim protocol above).
+struct WaveScatter4 {
+    volatile float* t;
     int x0, y0, z0;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c, float d)
     {
-        lds_float* q = t + tile_id(i - x0, j - y0, k - z0);
-        __builtin_amdgcn_ds_faddf(q, a, 0, 0, false);
-        __builtin_amdgcn_ds_faddf(q + TILE, b, 0, 0, false);
-        __builtin_amdgcn_ds_faddf(q + 2 * TILE, c, 0, 0, false);
-        __builtin_amdgcn_ds_faddf(q + 3 * TILE, d, 0, 0, false);
+        volatile float* q = t + tile_id(i - x0, j - y0, k - z0);
+        const float qa = q[0], qb = q[TILE], qc = q[2 * TILE], qd = q[3 * TILE];
+        q[0] = qa + a;
+        q[TILE] = qb + b;
+        q[2 * TILE] = qc + c;
+        q[3 * TILE] = qd + d;
     }
 };
 
-struct LdsScatter3 {
-    lds_float* t;
+struct WaveScatter3 {
+    volatile float* t;
     int x0, y0, z0;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c)
     {
-        lds_float* q = t + tile_id(i - x0, j - y0, k - z0);
-        __builtin_amdgcn_ds_faddf(q, a, 0, 0, false);
-        __builtin_amdgcn_ds_faddf(q + TILE, b, 0, 0, false);
-        __builtin_amdgcn_ds_faddf(q + 2 * TILE, c, 0, 0, false);
+        volatile float* q = t + tile_id(i - x0, j - y0, k - z0);
+        const float qa = q[0], qb = q[TILE], qc = q[2 * TILE];
+        q[0] = qa + a;
+        q[TILE] = qb + b;
+        q[2 * TILE] = qc + c;
     }
 };
 
-// Gather accessors for the tile with a per-node fall-back to global memory for stencils that
-// left the tile (one code path: gathers have no atomics to keep apart).
+// Gather accessors for a workgroup tile with a per-node fall-back to global memory for
+// stencils that left the tile.
 struct TileGatherV {
+    const float* t;
+    int x0, y0, z0;
+    bool in_tile;
+    GatherVout fb;
+    __device__ __forceinline__ float3 get(int i, int j, int k) const
+    {
+        if (in_tile) {
+            const int id = tile_id(i - x0, j - y0, k - z0);
+            return make_float3(t[id], t[TILE + id], t[2 * TILE + id]);
+        }
+        return fb.get(i, j, k);
+    }
+};
+
+struct TileGatherVfromP {
     const float* t;
     int x0, y0, z0;
     bool in_tile;
@@ -92,7 +123,7 @@ struct TileGatherPbar {
     const float* t;
     int x0, y0, z0;
     bool in_tile;
-    GatherPbarFromP fb;
+    GatherPbar fb;
     __device__ __forceinline__ float4 get(int i, int j, int k) const
     {
         if (in_tile) {
@@ -115,35 +146,28 @@ __device__ __forceinline__ void load_particle_t(const float* __restrict__ st, lo
     for (int a = 0; a < 9; a++) q.F[a] = with_F ? st[(SF + a) * LP + p] : ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f);
 }
 
-// Grid-stride clear of a float4 buffer, spread over the workgroups of a chunk kernel.
-__device__ __forceinline__ void clear_slice(float4* __restrict__ buf, long n)
+__device__ __forceinline__ void zero_lds(float* t, int nfloats)
 {
-    if (!buf) return;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
-        buf[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int e = threadIdx.x; e < nfloats; e += WG) t[e] = 0.0f;
 }
 
-__device__ __forceinline__ void zero_tile(float* t, int planes)
+__device__ __forceinline__ int node_key(const Stencil& s, int x0, int y0, int z0)
 {
-    for (int e = threadIdx.x; e < planes * TILE; e += WG) t[e] = 0.0f;
+    return ((s.b[0] - x0) * TS + (s.b[1] - y0)) * TS + (s.b[2] - z0);
 }
 
-// Add the nonzero entries of an LDS tile (ncomp planes) into a float4 grid with one global
-// atomic per component: lane -> (node, component), so a wavefront covers 16 consecutive
-// float4 nodes.
-__device__ __forceinline__ void flush_tile(const float* t, int ncomp, float4* __restrict__ gi, int n, int x0, int y0,
-                                           int z0)
+// Sum the wave tiles (ncomp planes each, stride wstride floats) of the workgroup into the
+// chunk's staging slot: TNODES float4 in (lx, ly, lz) order, unused components 0.
+__device__ __forceinline__ void store_staging(const float* wt, int wstride, int ncomp, float4* __restrict__ slot)
 {
-    float* g = reinterpret_cast<float*>(gi);
-    for (int e = threadIdx.x; e < 4 * TNODES; e += WG) {
-        const int node = e >> 2, comp = e & 3;
-        if (comp >= ncomp) continue;
-        const int lx = node / (TS * TS), ly = (node / TS) % TS, lz = node % TS;
-        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-        if (gx < 0 || gy < 0 || gz < 0 || gx >= n || gy >= n || gz >= n) continue;
-        const float v = t[comp * TILE + tile_id(lx, ly, lz)];
-        if (v != 0.0f) atomicAdd(g + 4 * node_index(n, gx, gy, gz) + comp, v);
+    for (int e = threadIdx.x; e < TNODES; e += WG) {
+        const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+        const int id = tile_id(lx, ly, lz);
+        float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int w = 0; w < WAVES; w++)
+            for (int c = 0; c < ncomp; c++) v[c] += wt[w * wstride + c * TILE + id];
+        slot[e] = make_float4(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -152,93 +176,149 @@ __device__ __forceinline__ void flush_tile(const float* t, int ncomp, float4* __
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
                                                   const Chunk* __restrict__ chunks, int nchunks,
-                                                  const double* __restrict__ params, float4* __restrict__ grid,
-                                                  DevScene sc, int P, unsigned* __restrict__ flag)
+                                                  const double* __restrict__ params, float4* __restrict__ staging,
+                                                  float4* __restrict__ fallback, DevScene sc, int P,
+                                                  unsigned* __restrict__ flag)
 {
-    __shared__ float tl[4 * TILE];
+    __shared__ float wt[WAVES * 4 * TILE];
+    __shared__ int own[WAVES * TNODES];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    volatile int* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        zero_tile(tl, 4);
+        zero_lds(wt, WAVES * 4 * TILE);
         __syncthreads();
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
-        float4* gi = grid + (long)ch.inst * nn;
         const InstParams ip = load_params(sc, params, ch.inst);
-#pragma unroll
+        WaveScatter4 acc{wt + wave * 4 * TILE, x0, y0, z0};
+#pragma unroll 1
         for (int r = 0; r < PPT; r++) {
             const int idx = r * WG + threadIdx.x;
-            if (idx >= ch.count) break;
-            const int p = ch.start + idx;
-            ParticleIn q;
-            load_particle_t(in, LP, p, ch.species == LIQUID, q);
+            const bool valid = idx < ch.count;
             Stencil s;
-            if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
-            float Fn[9];
             P2GPayload pl;
-            p2g_prepare(sc, ch.species, ip, q, Fn, pl);
-            if (stencil_in_tile(s, x0, y0, z0)) {
-                LdsScatter4 acc{(lds_float*)tl, x0, y0, z0};
-                p2g_scatter(sc, s, pl, acc);
-            } else {
-                ScatterGlobal4 acc{gi, n};
-                p2g_scatter(sc, s, pl, acc);
-            }
-            if (fout && ch.species == LIQUID) {
+            bool it = false;
+            if (valid) {
+                const int p = ch.start + idx;
+                ParticleIn q;
+                load_particle_t(in, LP, p, ch.species == LIQUID, q);
+                if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
+                float Fn[9];
+                p2g_prepare(sc, ch.species, ip, q, Fn, pl);
+                if (fout && ch.species == LIQUID) {
 #pragma unroll
-                for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
+                    for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
+                }
+                it = stencil_in_tile(s, x0, y0, z0);
+                if (!it) {
+                    ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
+                    p2g_scatter(sc, s, pl, g);
+                }
+            }
+            bool pending = valid && it;
+            const int key = pending ? node_key(s, x0, y0, z0) : 0;
+            while (__ballot(pending)) {
+                if (pending) ow[key] = lane;
+                if (pending && ow[key] == lane) {
+                    p2g_scatter(sc, s, pl, acc);
+                    pending = false;
+                }
             }
         }
         __syncthreads();
-        flush_tile(tl, 4, gi, n, x0, y0, z0);
+        store_staging(wt, 4 * TILE, 4, staging + (long)c * TNODES);
         __syncthreads();
     }
 }
 
-// Velocity tile: the grid update applied to the momentum/mass nodes of the tile.
-__device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __restrict__ gi, const DevScene& sc,
-                                                   int x0, int y0, int z0)
+// One thread per grid node (a workgroup of 64 = one 4x4x4 node block): the sum of the staging
+// slots of the chunks whose tiles cover the node, in a fixed order, plus the fall-back grid
+// (read and cleared for the next step); then the grid update.  Writes the velocity (vout) and,
+// when pout is given, the momentum/mass.
+__global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ staging, const int* __restrict__ bnch,
+                                                    const int* __restrict__ cstart, float4* __restrict__ fallback,
+                                                    float4* __restrict__ pout, float4* __restrict__ vout, DevScene sc,
+                                                    int NB)
 {
     const int n = sc.n_grid;
-    for (int e = threadIdx.x; e < TNODES; e += WG) {
-        const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
-        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-        float v[3] = {0.0f, 0.0f, 0.0f};
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) {
-            NodeRec r;
-            gridop_node(sc, gi[node_index(n, gx, gy, gz)], gx, gy, gz, v, r);
+    const long nn = (long)n * n * n;
+    const int nb3 = NB * NB * NB;
+    const int inst = blockIdx.x / nb3, blk = blockIdx.x % nb3;
+    const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
+    const int lx = threadIdx.x >> 4, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x & 3;
+    const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
+    if (gx >= n || gy >= n || gz >= n) return;
+    const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
+    float4 acc = fallback[gid];
+    fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // covering buckets per axis: local node 0..2 -> blocks b-1, b; local 3 -> b, b+1
+    const int cx0 = lx < 3 ? bx - 1 : bx, cy0 = ly < 3 ? by - 1 : by, cz0 = lz < 3 ? bz - 1 : bz;
+    for (int sp = 0; sp < 3; sp++) {
+        for (int dx = 0; dx < 2; dx++) {
+            const int cx = cx0 + dx;
+            if (cx < 0 || cx >= NB) continue;
+            for (int dy = 0; dy < 2; dy++) {
+                const int cy = cy0 + dy;
+                if (cy < 0 || cy >= NB) continue;
+                for (int dz = 0; dz < 2; dz++) {
+                    const int cz = cz0 + dz;
+                    if (cz < 0 || cz >= NB) continue;
+                    const int bucket = ((inst * 3 + sp) * NB + cx) * NB * NB + cy * NB + cz;
+                    const int nc = bnch[bucket];
+                    if (nc == 0) continue;
+                    const int e = ((gx - (cx * TB - 1)) * TS + (gy - (cy * TB - 1))) * TS + (gz - (cz * TB - 1));
+                    const float4* sl = staging + (long)cstart[bucket] * TNODES + e;
+                    for (int k = 0; k < nc; k++) {
+                        const float4 v = sl[(long)k * TNODES];
+                        acc.x += v.x;
+                        acc.y += v.y;
+                        acc.z += v.z;
+                        acc.w += v.w;
+                    }
+                }
+            }
         }
-        const int id = tile_id(lx, ly, lz);
-        tv[id] = v[0];
-        tv[TILE + id] = v[1];
-        tv[2 * TILE + id] = v[2];
     }
+    if (pout) pout[gid] = acc;
+    float v[3];
+    NodeRec r;
+    gridop_node(sc, acc, gx, gy, gz, v, r);
+    vout[gid] = make_float4(v[0], v[1], v[2], 0.0f);
 }
 
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
                                                   const Chunk* __restrict__ chunks, int nchunks,
-                                                  const float4* __restrict__ grid, float4* __restrict__ clear_next,
-                                                  long clear_n, const float* __restrict__ traj, int T, int f,
-                                                  double* __restrict__ dvsum, DevScene sc, int P,
+                                                  const float4* __restrict__ vgrid, const float* __restrict__ traj,
+                                                  int T, int f, double* __restrict__ dvsum, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
 {
     __shared__ float tv[3 * TILE];
-    __shared__ double red[WG / 64][3];
-    clear_slice(clear_next, clear_n);
+    __shared__ double red[WAVES][3];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
-        const float4* gi = grid + (long)ch.inst * nn;
-        load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        const float4* vi = vgrid + (long)ch.inst * nn;
+        for (int e = threadIdx.x; e < TNODES; e += WG) {
+            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) v = vi[node_index(n, gx, gy, gz)];
+            const int id = tile_id(lx, ly, lz);
+            tv[id] = v.x;
+            tv[TILE + id] = v.y;
+            tv[2 * TILE + id] = v.z;
+        }
         __syncthreads();
         double dv[3] = {0.0, 0.0, 0.0};
         const float* tr = traj + ((long)ch.inst * T + f) * 3;
-#pragma unroll
+#pragma unroll 1
         for (int r = 0; r < PPT; r++) {
             const int idx = r * WG + threadIdx.x;
             if (idx >= ch.count) break;
@@ -249,7 +329,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
             Stencil s;
             if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
             float nv[3], nC[9];
-            TileGatherV acc{tv, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherVfromP{gi, sc}};
+            TileGatherV acc{tv, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherVout{vi, n}};
             g2p_gather(sc, s, acc, nv, nC);
             if (ch.species == LIQUID) {
 #pragma unroll
@@ -286,7 +366,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
             __syncthreads();
             if (threadIdx.x < 3) {
                 double acc2 = 0.0;
-                for (int k = 0; k < WG / 64; k++) acc2 += red[k][threadIdx.x];
+                for (int k = 0; k < WAVES; k++) acc2 += red[k][threadIdx.x];
                 atomicAdd(dvsum + ((long)ch.inst * T + f + 1) * 3 + threadIdx.x, acc2);
             }
         }
@@ -296,27 +376,49 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
 
 // ---------------------------------------------------------------------------- adjoint
 
+// Velocity tile from the momentum/mass grid: the grid update at every node of the tile.
+__device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __restrict__ gi, const DevScene& sc,
+                                                   int x0, int y0, int z0)
+{
+    const int n = sc.n_grid;
+    for (int e = threadIdx.x; e < TNODES; e += WG) {
+        const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) {
+            NodeRec r;
+            gridop_node(sc, gi[node_index(n, gx, gy, gz)], gx, gy, gz, v, r);
+        }
+        const int id = tile_id(lx, ly, lz);
+        tv[id] = v[0];
+        tv[TILE + id] = v[1];
+        tv[2 * TILE + id] = v[2];
+    }
+}
+
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ st, const float* __restrict__ adj,
                                                       const Chunk* __restrict__ chunks, int nchunks,
                                                       const float4* __restrict__ grid,
                                                       const double* __restrict__ dvbar, int T, int f,
-                                                      float4* __restrict__ vbar, float* __restrict__ xbar,
-                                                      DevScene sc, int P)
+                                                      float4* __restrict__ staging, float4* __restrict__ fallback,
+                                                      float* __restrict__ xbar, DevScene sc, int P)
 {
     __shared__ float tv[3 * TILE];
-    __shared__ float tb[3 * TILE];
+    __shared__ float wt[WAVES * 3 * TILE];
+    __shared__ int own[WAVES * TNODES];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    volatile int* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const float4* gi = grid + (long)ch.inst * nn;
-        float4* bi = vbar + (long)ch.inst * nn;
         const bool liq = ch.species == LIQUID;
         if (liq) load_velocity_tile(tv, gi, sc, x0, y0, z0);
-        zero_tile(tb, 3);
+        zero_lds(wt, WAVES * 3 * TILE);
         __syncthreads();
         float vstick[3] = {0.0f, 0.0f, 0.0f};
         if (ch.species == STICK) {
@@ -324,98 +426,138 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
 #pragma unroll
             for (int a = 0; a < 3; a++) vstick[a] = -(float)d[a];
         }
-#pragma unroll
+        WaveScatter3 acc{wt + wave * 3 * TILE, x0, y0, z0};
+#pragma unroll 1
         for (int r = 0; r < PPT; r++) {
             const int idx = r * WG + threadIdx.x;
-            if (idx >= ch.count) break;
-            const int p = ch.start + idx;
-            float x[3];
-#pragma unroll
-            for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
+            const bool valid = idx < ch.count;
             Stencil s;
-            stencil(sc, x[0], x[1], x[2], s);
-            const bool it = stencil_in_tile(s, x0, y0, z0);
-            float Cb[9];
+            float Cb[9], vt[3] = {vstick[0], vstick[1], vstick[2]};
+            bool it = false;
+            if (valid) {
+                const int p = ch.start + idx;
+                float x[3];
 #pragma unroll
-            for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
-            float vt[3] = {vstick[0], vstick[1], vstick[2]}, xa[3] = {0.0f, 0.0f, 0.0f};
-            if (liq) {
+                for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
+                stencil(sc, x[0], x[1], x[2], s);
+                it = stencil_in_tile(s, x0, y0, z0);
 #pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    xa[a] = adj[(SX + a) * LP + p];
-                    vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
+                for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
+                if (liq) {
+                    float xa[3], xb[3];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        xa[a] = adj[(SX + a) * LP + p];
+                        vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
+                    }
+                    TileGatherVfromP gv{tv, x0, y0, z0, it, GatherVfromP{gi, sc}};
+                    g2p_adj_xgrad(sc, s, vt, Cb, gv, xb);
+#pragma unroll
+                    for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
+                }
+                if (!it) {
+                    ScatterGlobal3 g{fallback + (long)ch.inst * nn, n};
+                    g2p_adj_scatter(sc, s, vt, Cb, g);
                 }
             }
-            if (it) {
-                LdsScatter3 acc{(lds_float*)tb, x0, y0, z0};
-                g2p_adj_scatter(sc, s, vt, Cb, acc);
-            } else {
-                ScatterGlobal3 acc{bi, n};
-                g2p_adj_scatter(sc, s, vt, Cb, acc);
-            }
-            if (liq) {
-                float xb[3];
-                TileGatherV gv{tv, x0, y0, z0, it, GatherVfromP{gi, sc}};
-                g2p_adj_xgrad(sc, s, vt, Cb, gv, xb);
-#pragma unroll
-                for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
+            bool pending = valid && it;
+            const int key = pending ? node_key(s, x0, y0, z0) : 0;
+            while (__ballot(pending)) {
+                if (pending) ow[key] = lane;
+                if (pending && ow[key] == lane) {
+                    g2p_adj_scatter(sc, s, vt, Cb, acc);
+                    pending = false;
+                }
             }
         }
         __syncthreads();
-        flush_tile(tb, 3, bi, n, x0, y0, z0);
+        store_staging(wt, 3 * TILE, 3, staging + (long)c * TNODES);
         __syncthreads();
     }
 }
 
-// Adjoint-of-P2G input tile: the grid update's adjoint at every node of the tile.
-__device__ __forceinline__ void load_pbar_tile(float* tq, const float4* __restrict__ gi, const float4* __restrict__ bi,
-                                               const DevScene& sc, int x0, int y0, int z0)
+// One thread per grid node: dL/dv at the node (staging slots of the covering chunks plus the
+// fall-back grid, cleared for the next step), then the adjoint of the grid update with the
+// node's momentum/mass -> dL/d(momentum, mass) into pbar.
+__global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ staging, const int* __restrict__ bnch,
+                                                    const int* __restrict__ cstart, float4* __restrict__ fallback,
+                                                    const float4* __restrict__ grid, float4* __restrict__ pbar,
+                                                    DevScene sc, int NB)
 {
     const int n = sc.n_grid;
-    for (int e = threadIdx.x; e < TNODES; e += WG) {
-        const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
-        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-        float4 pb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) {
-            const long id = node_index(n, gx, gy, gz);
-            float v[3];
-            NodeRec r;
-            gridop_node(sc, gi[id], gx, gy, gz, v, r);
-            const float4 b = bi[id];
-            pb = gridop_node_adj(r, b.x, b.y, b.z);
+    const long nn = (long)n * n * n;
+    const int nb3 = NB * NB * NB;
+    const int inst = blockIdx.x / nb3, blk = blockIdx.x % nb3;
+    const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
+    const int lx = threadIdx.x >> 4, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x & 3;
+    const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
+    if (gx >= n || gy >= n || gz >= n) return;
+    const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
+    float4 acc = fallback[gid];
+    fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int cx0 = lx < 3 ? bx - 1 : bx, cy0 = ly < 3 ? by - 1 : by, cz0 = lz < 3 ? bz - 1 : bz;
+    for (int sp = 0; sp < 3; sp++) {
+        for (int dx = 0; dx < 2; dx++) {
+            const int cx = cx0 + dx;
+            if (cx < 0 || cx >= NB) continue;
+            for (int dy = 0; dy < 2; dy++) {
+                const int cy = cy0 + dy;
+                if (cy < 0 || cy >= NB) continue;
+                for (int dz = 0; dz < 2; dz++) {
+                    const int cz = cz0 + dz;
+                    if (cz < 0 || cz >= NB) continue;
+                    const int bucket = ((inst * 3 + sp) * NB + cx) * NB * NB + cy * NB + cz;
+                    const int nc = bnch[bucket];
+                    if (nc == 0) continue;
+                    const int e = ((gx - (cx * TB - 1)) * TS + (gy - (cy * TB - 1))) * TS + (gz - (cz * TB - 1));
+                    const float4* sl = staging + (long)cstart[bucket] * TNODES + e;
+                    for (int k = 0; k < nc; k++) {
+                        const float4 v = sl[(long)k * TNODES];
+                        acc.x += v.x;
+                        acc.y += v.y;
+                        acc.z += v.z;
+                    }
+                }
+            }
         }
-        const int id = tile_id(lx, ly, lz);
-        tq[id] = pb.x;
-        tq[TILE + id] = pb.y;
-        tq[2 * TILE + id] = pb.z;
-        tq[3 * TILE + id] = pb.w;
     }
+    float v[3];
+    NodeRec r;
+    gridop_node(sc, grid[gid], gx, gy, gz, v, r);
+    pbar[gid] = gridop_node_adj(r, acc.x, acc.y, acc.z);
 }
 
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, float* __restrict__ adj,
                                                       const float* __restrict__ xbar, const Chunk* __restrict__ chunks,
                                                       int nchunks, const double* __restrict__ params,
-                                                      const float4* __restrict__ grid, const float4* __restrict__ vbar,
-                                                      float4* __restrict__ clear_next, long clear_n,
-                                                      double* __restrict__ thbar, DevScene sc, int P)
+                                                      const float4* __restrict__ pbar, double* __restrict__ thbar,
+                                                      DevScene sc, int P)
 {
     __shared__ float tq[4 * TILE];
-    __shared__ double red[WG / 64][4];
-    clear_slice(clear_next, clear_n);
+    __shared__ double red[WAVES][4];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
-        const float4* gi = grid + (long)ch.inst * nn;
-        const float4* bi = vbar + (long)ch.inst * nn;
-        load_pbar_tile(tq, gi, bi, sc, x0, y0, z0);
+        const float4* qi = pbar + (long)ch.inst * nn;
+        for (int e = threadIdx.x; e < TNODES; e += WG) {
+            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) v = qi[node_index(n, gx, gy, gz)];
+            const int id = tile_id(lx, ly, lz);
+            tq[id] = v.x;
+            tq[TILE + id] = v.y;
+            tq[2 * TILE + id] = v.z;
+            tq[3 * TILE + id] = v.w;
+        }
         __syncthreads();
         double tb[4] = {0.0, 0.0, 0.0, 0.0};
         const InstParams ip = load_params(sc, params, ch.inst);
-#pragma unroll
+#pragma unroll 1
         for (int r = 0; r < PPT; r++) {
             const int idx = r * WG + threadIdx.x;
             if (idx >= ch.count) break;
@@ -430,7 +572,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
 #pragma unroll
                 for (int a = 0; a < 3; a++) xp[a] = xbar[a * LP + p];
-                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
+                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbar{qi, n}};
                 p2g_adj_liquid(sc, ip, q, s, Fnb, xp, acc, xo, vo, Co, Fo, tb);
 #pragma unroll
                 for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
@@ -447,7 +589,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 Stencil s;
                 stencil(sc, x[0], x[1], x[2], s);
                 float Co[9];
-                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbarFromP{gi, bi, sc}};
+                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbar{qi, n}};
                 p2g_adj_rigid(sc, species_mass(sc, ch.species), s, acc, Co);
 #pragma unroll
                 for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
@@ -465,7 +607,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             __syncthreads();
             if (threadIdx.x < 4) {
                 double a2 = 0.0;
-                for (int k = 0; k < WG / 64; k++) a2 += red[k][threadIdx.x];
+                for (int k = 0; k < WAVES; k++) a2 += red[k][threadIdx.x];
                 atomicAdd(thbar + 4 * ch.inst + threadIdx.x, a2);
             }
         }
