@@ -18,8 +18,8 @@
 //   * before scattering, the lanes of a wavefront claim their stencil base node in a per-wave
 //     table; lanes that share a base are served in successive passes, so within one pass the
 //     27 read-modify-writes of different lanes never hit the same node in the same
-//     instruction, and a wavefront's LDS operations complete in program order (volatile
-//     accesses keep the compiler from reordering them);
+//     instruction, and a wavefront's LDS operations complete in program order (compiler
+//     memory barriers keep the compiler from reordering them);
 //   * the workgroup sums its four wave tiles and stores the sum in the chunk's staging slot
 //     (8 KB of plain coalesced stores);
 //   * a reduction kernel, one thread per grid node, adds the staging slots of the (at most
@@ -59,31 +59,39 @@ __device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0
     return a <= (unsigned)(TS - 3) && b <= (unsigned)(TS - 3) && c <= (unsigned)(TS - 3);
 }
 
+// Compiler-only memory barrier: keeps LDS accesses in program order without waiting for
+// them (a wavefront's LDS operations execute in order in hardware).
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
 // Read-modify-write scatter into a wavefront's private tile (see the claim protocol above).
+// The barrier after each node keeps the next node's reads behind this node's writes: another
+// lane's stencil may reach the same node at a different offset.
 struct WaveScatter4 {
-    volatile float* t;
+    float* t;
     int x0, y0, z0;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c, float d)
     {
-        volatile float* q = t + tile_id(i - x0, j - y0, k - z0);
+        float* q = t + tile_id(i - x0, j - y0, k - z0);
         const float qa = q[0], qb = q[TILE], qc = q[2 * TILE], qd = q[3 * TILE];
         q[0] = qa + a;
         q[TILE] = qb + b;
         q[2 * TILE] = qc + c;
         q[3 * TILE] = qd + d;
+        lds_order();
     }
 };
 
 struct WaveScatter3 {
-    volatile float* t;
+    float* t;
     int x0, y0, z0;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c)
     {
-        volatile float* q = t + tile_id(i - x0, j - y0, k - z0);
+        float* q = t + tile_id(i - x0, j - y0, k - z0);
         const float qa = q[0], qb = q[TILE], qc = q[2 * TILE];
         q[0] = qa + a;
         q[TILE] = qb + b;
         q[2 * TILE] = qc + c;
+        lds_order();
     }
 };
 
@@ -186,7 +194,7 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
     const long nn = (long)n * n * n;
     const long LP = P;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    volatile int* ow = own + wave * TNODES;
+    int* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         zero_lds(wt, WAVES * 4 * TILE);
         __syncthreads();
@@ -221,7 +229,9 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
             bool pending = valid && it;
             const int key = pending ? node_key(s, x0, y0, z0) : 0;
             while (__ballot(pending)) {
+                lds_order();
                 if (pending) ow[key] = lane;
+                lds_order();
                 if (pending && ow[key] == lane) {
                     p2g_scatter(sc, s, pl, acc);
                     pending = false;
@@ -411,7 +421,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
     const long nn = (long)n * n * n;
     const long LP = P;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    volatile int* ow = own + wave * TNODES;
+    int* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
@@ -463,7 +473,9 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
             bool pending = valid && it;
             const int key = pending ? node_key(s, x0, y0, z0) : 0;
             while (__ballot(pending)) {
+                lds_order();
                 if (pending) ow[key] = lane;
+                lds_order();
                 if (pending && ow[key] == lane) {
                     g2p_adj_scatter(sc, s, vt, Cb, acc);
                     pending = false;
