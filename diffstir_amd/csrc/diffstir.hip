@@ -78,6 +78,7 @@ struct ds_ctx {
     float4* d_staging = nullptr;   // per-chunk staging slots
     long staging_cap = 0;          // slots allocated
     float4* d_seg_grid = nullptr;
+    float* d_seg_aff = nullptr;    // K x 9 x P liquid affine A per step of a segment
     float* d_adj2 = nullptr;
     int* d_meta2 = nullptr;
     int* d_pid = nullptr;
@@ -231,6 +232,7 @@ void free_mode(ds_ctx* c)
     dfree(c->d_staging);
     c->staging_cap = 0;
     dfree(c->d_seg_grid);
+    dfree(c->d_seg_aff);
     dfree(c->d_adj2);
     dfree(c->d_meta2);
     dfree(c->d_pid);
@@ -302,6 +304,7 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_pfb, G);
         MALLOC(c->d_vbfb, G);
         MALLOC(c->d_seg_grid, (size_t)G * c->K);
+        MALLOC(c->d_seg_aff, (size_t)9 * c->P * c->K);
         MALLOC(c->d_adj2, SF);
         MALLOC(c->d_meta2, c->P);
         MALLOC(c->d_pid, c->P);
@@ -447,13 +450,13 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
 
 // P2G of a checkpoint segment's chunks into the staging slots, then the grid reduction:
 // velocity into d_vgrid and, when pout is given, momentum/mass into pout.
-void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* pout)
+void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* pout, float* aout)
 {
     const int nch = c->nch_h[cp];
     const int* bnch = c->d_bnch + (size_t)cp * c->NBK;
     const int* cstart = c->d_cstart + (size_t)cp * c->NBK;
     if (nch > 0)
-        DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout,
+        DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
                         c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, c->d_staging, c->d_pfb, c->dsc, c->P,
                         c->d_flag);
     const int nb3 = c->NB * c->NB * c->NB;
@@ -481,7 +484,7 @@ int forward_tiled(ds_ctx* c, int nsteps)
             RCHK(sort_into_checkpoint(c, cp));
             in = c->d_ckpt + (size_t)cp * SF;
         }
-        p2g_reduce_tiled(c, in, c->d_state, cp, nullptr);
+        p2g_reduce_tiled(c, in, c->d_state, cp, nullptr, nullptr);
         g2p_tiled(c, in, c->d_state, cp, s, true);
         c->step++;
     }
@@ -505,10 +508,11 @@ int backward_tiled(ds_ctx* c)
         const float* ck = c->d_ckpt + (size_t)cp * SF;
         auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : c->d_seg + (size_t)k * SF; };
         auto grid_of = [&](int k) -> float4* { return c->d_seg_grid + (size_t)k * G; };
+        auto aff_of = [&](int k) -> float* { return c->d_seg_aff + (size_t)k * 9 * c->P; };
         // recompute the segment, keeping each step's state and momentum/mass grid
         for (int k = 0; k < L; k++) {
             const int s = s0 + k;
-            p2g_reduce_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k));
+            p2g_reduce_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k), aff_of(k));
             if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, s, false);
         }
         // adjoint, last step first
@@ -521,8 +525,8 @@ int backward_tiled(ds_ctx* c)
             hipLaunchKernelGGL(k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
                                c->d_vbfb, grid_of(k), c->d_pbar2, c->dsc, c->NB);
             if (nch > 0)
-                DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
-                                c->d_xbar, chunks, nch, c->d_params, c->d_pbar2, c->d_thbar, c->dsc, c->P);
+                DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), aff_of(k),
+                                c->d_adj, c->d_xbar, chunks, nch, c->d_params, c->d_pbar2, c->d_thbar, c->dsc, c->P);
         }
         if (cp > 0) {
             hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,

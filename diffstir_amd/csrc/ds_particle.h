@@ -215,23 +215,23 @@ __device__ __forceinline__ void g2p_adj_particle(const DevScene& sc, const Stenc
     else xb[0] = xb[1] = xb[2] = 0.0f;
 }
 
-// Adjoint of P2G for one liquid particle.  Gathers (dL/dmomentum, dL/dmass); Fnb_in is the
-// adjoint of F_tmp from the next step's F; xb_partial the G2P part of dL/dx.  Writes the
-// adjoints of x, v, C, F and adds dL/dparams into tb.
+// Adjoint of P2G for one liquid particle, gather half: with the scattered affine A and
+// u = v + dt g, gathers (dL/dmomentum, dL/dmass) and accumulates ub = dL/du, Ab = dL/dA and the
+// weight/offset part of dL/dfx.
 template <class Gather>
-__device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstParams& ip, const ParticleIn& q,
-                                               const Stencil& s, const float Fnb_in[9], const float xb_partial[3],
-                                               Gather& acc, float xo[3], float vo[3], float Co[9], float Fo[9],
-                                               double tb[4])
+__device__ __forceinline__ void p2g_adj_gather(const DevScene& sc, const Stencil& s, const float A[9], const float u[3],
+                                               Gather& acc, float ub[3], float Ab[9], float fxb[3])
 {
-    LiquidP2G L;
-    liquid_p2g(sc, ip.th, ip.kb, q.C, q.F, L);
     const float m = sc.m_liq;
-    const float u[3] = {q.v[0], q.v[1] + sc.gdt, q.v[2]};
-    float ub[3] = {0.0f, 0.0f, 0.0f}, Ab[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float fxb[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; a++) ub[a] = fxb[a] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 9; a++) Ab[a] = 0.0f;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
+        // one x-slice of nodes at a time: keeps the compiler from hoisting all 27 gathers
+        // (and their registers) ahead of the arithmetic
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int j = 0; j < 3; j++) {
 #pragma unroll
@@ -245,8 +245,7 @@ __device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstPar
                 const float gg[3] = {gq.x, gq.y, gq.z};
                 float mom[3];
 #pragma unroll
-                for (int a = 0; a < 3; a++)
-                    mom[a] = m * u[a] + L.A[a * 3] * dp[0] + L.A[a * 3 + 1] * dp[1] + L.A[a * 3 + 2] * dp[2];
+                for (int a = 0; a < 3; a++) mom[a] = m * u[a] + A[a * 3] * dp[0] + A[a * 3 + 1] * dp[1] + A[a * 3 + 2] * dp[2];
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     ub[a] += W * m * gg[a];
@@ -256,12 +255,25 @@ __device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstPar
                 const float Wb = gg[0] * mom[0] + gg[1] * mom[1] + gg[2] * mom[2] + gq.w * m;
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const float atg = L.A[0 * 3 + c] * gg[0] + L.A[1 * 3 + c] * gg[1] + L.A[2 * 3 + c] * gg[2];
+                    const float atg = A[0 * 3 + c] * gg[0] + A[1 * 3 + c] * gg[1] + A[2 * 3 + c] * gg[2];
                     fxb[c] += Wb * dW[c] - sc.dx * W * atg;
                 }
             }
         }
     }
+}
+
+// Adjoint of P2G for one liquid particle, stress half: from Ab = dL/dA, back through
+// A = alpha S + m C, S = eta(|C|) sym(C) + I (k/2)(J - 1/J - 1) J, J = S0 S1 S2 of the FP64 SVD
+// of F_tmp (simulation_functions.py:111-153) and F_tmp = (I + dt C) F.  L holds the particle's
+// P2G quantities (liquid_p2g); Fnb_in is the adjoint of F_tmp from the next step's F and
+// xb_partial the G2P part of dL/dx.  Writes the adjoints of x, v, C, F and adds dL/dparams into tb.
+__device__ __forceinline__ void p2g_adj_stress(const DevScene& sc, const InstParams& ip, const ParticleIn& q,
+                                               const LiquidP2G& L, const float ub[3], const float Ab[9],
+                                               const float fxb[3], const float Fnb_in[9], const float xb_partial[3],
+                                               float xo[3], float vo[3], float Co[9], float Fo[9], double tb[4])
+{
+    const float m = sc.m_liq;
     const float* C = q.C;
     float Cb[9], Sb[9];
 #pragma unroll
@@ -269,7 +281,6 @@ __device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstPar
         Cb[a] = m * Ab[a];
         Sb[a] = sc.alpha * Ab[a];
     }
-    // S = eta sym(C) + I (k/2)(J - 1/J - 1) J
     float etab = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++)
@@ -323,6 +334,21 @@ __device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstPar
     for (int a = 0; a < 3; a++) vo[a] = ub[a];
 #pragma unroll
     for (int a = 0; a < 9; a++) Co[a] = Cb[a];
+}
+
+// Adjoint of P2G for one liquid particle (gather and stress halves, P2G quantities computed here).
+template <class Gather>
+__device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstParams& ip, const ParticleIn& q,
+                                               const Stencil& s, const float Fnb_in[9], const float xb_partial[3],
+                                               Gather& acc, float xo[3], float vo[3], float Co[9], float Fo[9],
+                                               double tb[4])
+{
+    LiquidP2G L;
+    liquid_p2g(sc, ip.th, ip.kb, q.C, q.F, L);
+    const float u[3] = {q.v[0], q.v[1] + sc.gdt, q.v[2]};
+    float ub[3], Ab[9], fxb[3];
+    p2g_adj_gather(sc, s, L.A, u, acc, ub, Ab, fxb);
+    p2g_adj_stress(sc, ip, q, L, ub, Ab, fxb, Fnb_in, xb_partial, xo, vo, Co, Fo, tb);
 }
 
 // Adjoint of P2G for a stick or container particle: only C carries an adjoint

@@ -183,7 +183,8 @@ __device__ __forceinline__ void store_staging(const float* wt, int wstride, int 
 
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
-                                                  const Chunk* __restrict__ chunks, int nchunks,
+                                                  float* __restrict__ aout, const Chunk* __restrict__ chunks,
+                                                  int nchunks,
                                                   const double* __restrict__ params, float4* __restrict__ staging,
                                                   float4* __restrict__ fallback, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
@@ -220,6 +221,10 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
 #pragma unroll
                     for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
                 }
+                if (aout && ch.species == LIQUID) {
+#pragma unroll
+                    for (int a = 0; a < 9; a++) aout[a * LP + p] = pl.A[a];
+                }
                 it = stencil_in_tile(s, x0, y0, z0);
                 if (!it) {
                     ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
@@ -244,6 +249,59 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
     }
 }
 
+// The chunk counts and first chunks of the 27 buckets around node block (bx, by, bz), all
+// three species, into LDS (index sp * 27 + (dx+1) * 9 + (dy+1) * 3 + (dz+1); 0 chunks outside
+// the grid).
+__device__ __forceinline__ void load_neighbour_buckets(int* s_nc, int* s_cs, const int* __restrict__ bnch,
+                                                       const int* __restrict__ cstart, int inst, int bx, int by,
+                                                       int bz, int NB)
+{
+    for (int e = threadIdx.x; e < 3 * 27; e += blockDim.x) {
+        const int sp = e / 27, r = e % 27;
+        const int cx = bx + r / 9 - 1, cy = by + (r / 3) % 3 - 1, cz = bz + r % 3 - 1;
+        int nc = 0, cs = 0;
+        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < NB && cy < NB && cz < NB) {
+            const int bucket = ((inst * 3 + sp) * NB + cx) * NB * NB + cy * NB + cz;
+            nc = bnch[bucket];
+            cs = cstart[bucket];
+        }
+        s_nc[e] = nc;
+        s_cs[e] = cs;
+    }
+}
+
+// Sum, in a fixed order, the staging entries of node (local lx, ly, lz of its 4x4x4 node
+// block) from every chunk whose tile covers it.  Per axis the covering buckets are b-1 and b
+// for local nodes 0..2, b and b+1 for local node 3; the node's index in the tile of bucket
+// b + d is local - 4d + 1.
+__device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __restrict__ staging, const int* s_nc,
+                                                   const int* s_cs, int lx, int ly, int lz, int ncomp)
+{
+    const int ox = lx < 3 ? -1 : 0, oy = ly < 3 ? -1 : 0, oz = lz < 3 ? -1 : 0;
+#pragma unroll
+    for (int sp = 0; sp < 3; sp++)
+#pragma unroll
+        for (int ddx = 0; ddx < 2; ddx++)
+#pragma unroll
+            for (int ddy = 0; ddy < 2; ddy++)
+#pragma unroll
+                for (int ddz = 0; ddz < 2; ddz++) {
+                    const int dx = ox + ddx, dy = oy + ddy, dz = oz + ddz;
+                    const int r = sp * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+                    const int nc = s_nc[r];
+                    if (nc == 0) continue;
+                    const int e = ((lx - 4 * dx + 1) * TS + (ly - 4 * dy + 1)) * TS + (lz - 4 * dz + 1);
+                    const float4* sl = staging + (long)s_cs[r] * TNODES + e;
+                    for (int k = 0; k < nc; k++) {
+                        const float4 v = sl[(long)k * TNODES];
+                        acc.x += v.x;
+                        acc.y += v.y;
+                        acc.z += v.z;
+                        if (ncomp == 4) acc.w += v.w;
+                    }
+                }
+}
+
 // One thread per grid node (a workgroup of 64 = one 4x4x4 node block): the sum of the staging
 // slots of the chunks whose tiles cover the node, in a fixed order, plus the fall-back grid
 // (read and cleared for the next step); then the grid update.  Writes the velocity (vout) and,
@@ -260,38 +318,14 @@ __global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ s
     const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
     const int lx = threadIdx.x >> 4, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x & 3;
     const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
+    __shared__ int s_nc[3 * 27], s_cs[3 * 27];
+    load_neighbour_buckets(s_nc, s_cs, bnch, cstart, inst, bx, by, bz, NB);
+    __syncthreads();
     if (gx >= n || gy >= n || gz >= n) return;
     const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
     float4 acc = fallback[gid];
     fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    // covering buckets per axis: local node 0..2 -> blocks b-1, b; local 3 -> b, b+1
-    const int cx0 = lx < 3 ? bx - 1 : bx, cy0 = ly < 3 ? by - 1 : by, cz0 = lz < 3 ? bz - 1 : bz;
-    for (int sp = 0; sp < 3; sp++) {
-        for (int dx = 0; dx < 2; dx++) {
-            const int cx = cx0 + dx;
-            if (cx < 0 || cx >= NB) continue;
-            for (int dy = 0; dy < 2; dy++) {
-                const int cy = cy0 + dy;
-                if (cy < 0 || cy >= NB) continue;
-                for (int dz = 0; dz < 2; dz++) {
-                    const int cz = cz0 + dz;
-                    if (cz < 0 || cz >= NB) continue;
-                    const int bucket = ((inst * 3 + sp) * NB + cx) * NB * NB + cy * NB + cz;
-                    const int nc = bnch[bucket];
-                    if (nc == 0) continue;
-                    const int e = ((gx - (cx * TB - 1)) * TS + (gy - (cy * TB - 1))) * TS + (gz - (cz * TB - 1));
-                    const float4* sl = staging + (long)cstart[bucket] * TNODES + e;
-                    for (int k = 0; k < nc; k++) {
-                        const float4 v = sl[(long)k * TNODES];
-                        acc.x += v.x;
-                        acc.y += v.y;
-                        acc.z += v.z;
-                        acc.w += v.w;
-                    }
-                }
-            }
-        }
-    }
+    sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, 4);
     if (pout) pout[gid] = acc;
     float v[3];
     NodeRec r;
@@ -503,36 +537,14 @@ __global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ s
     const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
     const int lx = threadIdx.x >> 4, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x & 3;
     const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
+    __shared__ int s_nc[3 * 27], s_cs[3 * 27];
+    load_neighbour_buckets(s_nc, s_cs, bnch, cstart, inst, bx, by, bz, NB);
+    __syncthreads();
     if (gx >= n || gy >= n || gz >= n) return;
     const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
     float4 acc = fallback[gid];
     fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const int cx0 = lx < 3 ? bx - 1 : bx, cy0 = ly < 3 ? by - 1 : by, cz0 = lz < 3 ? bz - 1 : bz;
-    for (int sp = 0; sp < 3; sp++) {
-        for (int dx = 0; dx < 2; dx++) {
-            const int cx = cx0 + dx;
-            if (cx < 0 || cx >= NB) continue;
-            for (int dy = 0; dy < 2; dy++) {
-                const int cy = cy0 + dy;
-                if (cy < 0 || cy >= NB) continue;
-                for (int dz = 0; dz < 2; dz++) {
-                    const int cz = cz0 + dz;
-                    if (cz < 0 || cz >= NB) continue;
-                    const int bucket = ((inst * 3 + sp) * NB + cx) * NB * NB + cy * NB + cz;
-                    const int nc = bnch[bucket];
-                    if (nc == 0) continue;
-                    const int e = ((gx - (cx * TB - 1)) * TS + (gy - (cy * TB - 1))) * TS + (gz - (cz * TB - 1));
-                    const float4* sl = staging + (long)cstart[bucket] * TNODES + e;
-                    for (int k = 0; k < nc; k++) {
-                        const float4 v = sl[(long)k * TNODES];
-                        acc.x += v.x;
-                        acc.y += v.y;
-                        acc.z += v.z;
-                    }
-                }
-            }
-        }
-    }
+    sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, 3);
     float v[3];
     NodeRec r;
     gridop_node(sc, grid[gid], gx, gy, gz, v, r);
@@ -540,8 +552,9 @@ __global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ s
 }
 
 template <int PPT>
-__global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, float* __restrict__ adj,
-                                                      const float* __restrict__ xbar, const Chunk* __restrict__ chunks,
+__global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, const float* __restrict__ ain,
+                                                      float* __restrict__ adj, const float* __restrict__ xbar,
+                                                      const Chunk* __restrict__ chunks,
                                                       int nchunks, const double* __restrict__ params,
                                                       const float4* __restrict__ pbar, double* __restrict__ thbar,
                                                       DevScene sc, int P)
@@ -580,12 +593,24 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 Stencil s;
                 stencil(sc, q.x[0], q.x[1], q.x[2], s);
                 float Fnb[9], xp[3], xo[3], vo[3], Co[9], Fo[9];
+                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbar{qi, n}};
+                // gather with the affine stored by the recompute pass, then the FP64 SVD and the
+                // stress adjoint.  The empty asm ties C to the gather's results so the compiler
+                // cannot hoist the (independent) SVD into the gather and keep both live.
+                float A[9], ub[3], Ab[9], fxb[3];
+#pragma unroll
+                for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
+                const float u[3] = {q.v[0], q.v[1] + sc.gdt, q.v[2]};
+                p2g_adj_gather(sc, s, A, u, acc, ub, Ab, fxb);
+#pragma unroll
+                for (int a = 0; a < 9; a++) asm volatile("" : "+v"(q.C[a]) : "v"(Ab[a]), "v"(ub[a % 3]), "v"(fxb[a % 3]));
 #pragma unroll
                 for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
 #pragma unroll
                 for (int a = 0; a < 3; a++) xp[a] = xbar[a * LP + p];
-                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbar{qi, n}};
-                p2g_adj_liquid(sc, ip, q, s, Fnb, xp, acc, xo, vo, Co, Fo, tb);
+                LiquidP2G L;
+                liquid_p2g(sc, ip.th, ip.kb, q.C, q.F, L);
+                p2g_adj_stress(sc, ip, q, L, ub, Ab, fxb, Fnb, xp, xo, vo, Co, Fo, tb);
 #pragma unroll
                 for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
 #pragma unroll
