@@ -1,0 +1,40 @@
+"""`StirRollout`: the checkpointed rollout as a torch.autograd.Function.
+
+    forces = StirRollout.apply(params, engine)      # params [B, 4] float64, forces [B, T, 3]
+    loss = some_torch_function(forces); loss.backward()
+    params.grad                                      # dL/dparams from the HIP adjoint
+
+The forward runs ds_reset + ds_forward over the whole trajectory for every instance of the
+engine; the backward hands dL/dforces to the library's checkpointed adjoint.  Tensors live on
+the host (the force series and the parameters are a few numbers per instance); the particle
+state never leaves the GPU.  This replaces the reference's ti.ad.Tape over forward(s) calls
+(src/gradient_descent_module.py:103-107) for torch-side losses and optimisers.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .engine import Engine
+
+
+class StirRollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params: torch.Tensor, engine: Engine) -> torch.Tensor:
+        p = params.detach().to("cpu", torch.float64).numpy()
+        engine.set_params(p.reshape(engine.B, 4))
+        engine.reset()
+        engine.forward()
+        F = engine.forces()
+        ctx.engine = engine
+        return torch.from_numpy(F).to(dtype=torch.float64, device=params.device)
+
+    @staticmethod
+    def backward(ctx, grad_forces: torch.Tensor):
+        g = grad_forces.detach().to("cpu", torch.float64).numpy()
+        d = ctx.engine.backward(np.ascontiguousarray(g))
+        return torch.from_numpy(d).to(dtype=torch.float64, device=grad_forces.device), None
+
+
+def rollout(params: torch.Tensor, engine: Engine) -> torch.Tensor:
+    return StirRollout.apply(params, engine)

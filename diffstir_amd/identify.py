@@ -1,0 +1,75 @@
+"""Batched parameter identification: Adam in log-parameter space over B random restarts.
+
+Every restart is one instance of a single Engine, so one forward + adjoint launch sequence
+serves all restarts.  The target is a synthetic stick-force profile from the forward model at
+ground-truth parameters, optionally with Gaussian noise of a given relative size.  The loss
+per restart is the force MSE over x and z; the update is Adam on phi = log(theta) with
+dL/dphi = theta * dL/dtheta.  The reference fits with its own heuristic step rule instead
+(src/gradient_descent_module.py:11-42, see gradient_descent.parameters_fitting).
+"""
+from __future__ import annotations
+
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .engine import Engine
+from .loss import batch_loss
+from .scene import Scene
+
+
+def synthetic_target(scene: Scene, truth: Sequence[float], noise: float = 0.01, seed: int = 0,
+                     ckpt_every: int = 50) -> np.ndarray:
+    """Forces of `scene` at parameters `truth` plus Gaussian noise of `noise` x the RMS force."""
+    with Engine([scene], ckpt_every=ckpt_every) as e:
+        e.set_params(np.asarray(truth, dtype=np.float64)[None])
+        e.reset()
+        e.forward()
+        F = e.forces()[0]
+    rms = np.sqrt(np.mean(F[2:] ** 2)) if F.shape[0] > 2 else 1.0
+    rng = np.random.default_rng(seed)
+    return F + noise * rms * rng.normal(size=F.shape)
+
+
+def identify(scenes: Sequence[Scene], target: np.ndarray, init_params: np.ndarray, iterations: int = 100,
+             lr: float = 1e-2, fit_mask: Sequence[bool] = (True, True, True, False), betas=(0.9, 0.999),
+             eps: float = 1e-8, ckpt_every: int = 50, callback=None, engine: Optional[Engine] = None) -> dict:
+    """Adam on log-parameters for B restarts (one instance each) against one target profile.
+
+    Returns the per-iteration losses [iters, B], parameters [iters+1, B, 4] and the final
+    parameters."""
+    B = len(scenes)
+    theta = np.asarray(init_params, dtype=np.float64).reshape(B, 4).copy()
+    if np.any(theta[:, np.asarray(fit_mask)] <= 0):
+        raise ValueError("log-parameter fitting needs positive initial values for the fitted parameters")
+    mask = np.asarray(fit_mask, dtype=bool)
+    phi = np.where(mask, np.log(np.where(theta > 0, theta, 1.0)), 0.0)
+    m = np.zeros_like(phi)
+    v = np.zeros_like(phi)
+    targets = np.broadcast_to(np.asarray(target, dtype=np.float64), (B,) + np.asarray(target).shape)
+    own = engine is None
+    eng = engine if engine is not None else Engine(list(scenes), ckpt_every=ckpt_every)
+    losses, hist = [], [theta.copy()]
+    try:
+        for it in range(iterations):
+            eng.set_params(theta)
+            eng.reset()
+            eng.forward()
+            F = eng.forces()
+            L, g = batch_loss(F, targets, 1)
+            d = eng.backward(g)
+            gphi = np.where(mask, theta * d, 0.0)
+            m = betas[0] * m + (1 - betas[0]) * gphi
+            v = betas[1] * v + (1 - betas[1]) * gphi * gphi
+            mh = m / (1 - betas[0] ** (it + 1))
+            vh = v / (1 - betas[1] ** (it + 1))
+            phi = phi - lr * mh / (np.sqrt(vh) + eps)
+            theta = np.where(mask, np.exp(phi), theta)
+            losses.append(L)
+            hist.append(theta.copy())
+            if callback is not None:
+                callback(it, L, theta)
+    finally:
+        if own:
+            eng.close()
+    return {"losses": np.array(losses), "params": np.array(hist), "final": theta}

@@ -1,0 +1,123 @@
+"""The reference's driver API on the GPU: forward(s)/Tape/compute_loss, parameters_fitting, the
+force replay and export, the torch autograd wrapper and the batched identification loop."""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from diffstir_amd import scene as S
+from diffstir_amd import simulator as simmod
+from diffstir_amd.engine import Engine
+from diffstir_amd.force_calculation import force_calculation_and_animation
+from diffstir_amd.gradient_descent import parameters_fitting
+from diffstir_amd.identify import identify
+from diffstir_amd.loss import compute_loss
+from diffstir_amd.simulator import Simulator
+from oracle.oracle import Oracle, loss_and_grad
+from tests.helpers import force_err, grad_err, ref_traj
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tape_over_forward_calls_matches_oracle():
+    """gdm:103-107 restated: forward(f) for every substep inside the tape, compute_loss, and the
+    parameter gradient on exit."""
+    sc = S.tiny_scene(16, 20, seed=1)
+    print("[gpu] tape over forward(s)", flush=True)
+    sim = simmod.setup(sc, ckpt_every=6)
+    o = Oracle("f64")
+    Fo = o.rollout(sc)
+    sim.operator_forces_seq = 0.6 * Fo
+    with simmod.Tape():
+        simmod.initialize_ti_field()
+        for f in range(sc.max_steps - 1):
+            simmod.forward(f)
+        L = simmod.compute_loss()
+    Lo, go = loss_and_grad(Fo, 0.6 * Fo, sc.cfg.loss_kind)
+    _, dth_o = o.rollout_grad(sc, go, ckpt=6)
+    assert L == pytest.approx(Lo, rel=1e-4)
+    assert grad_err(sim.constitutive_parameters_grad, dth_o) <= 1e-3
+    with pytest.raises(ValueError):
+        simmod.forward(3)           # out of order
+    with pytest.raises(RuntimeError):
+        simmod.backward(0)          # only through the tape
+    sim.close()
+    simmod.SIM = None
+
+
+def test_parameters_fitting_runs_the_reference_epoch_loop():
+    sc = S.tiny_scene(16, 12, seed=2)
+    print("[gpu] parameters_fitting", flush=True)
+    sim = Simulator(sc, ckpt_every=6)
+    truth = sc.params.copy()
+    with Engine([sc], ckpt_every=6) as e:
+        e.reset()
+        e.forward()
+        sim.operator_forces_seq = e.forces()[0]
+    sim.constitutive_parameters[:3] = truth[:3] * np.array([1.5, 0.7, 1.2])
+    lo = [0.1, 0.0, 0.5, 0.0]
+    hi = [5.0, 20.0, 1.5, 1e4]
+    best, hist = parameters_fitting(sim, 2, lo, hi, n_fit=3)
+    assert len(hist["losses"]) == 6
+    assert np.all(np.isfinite(hist["losses"]))
+    assert min(hist["losses"]) <= hist["losses"][0]
+    for p, a, b in zip(best[:3], lo, hi):
+        assert a <= p <= b
+    sim.close()
+
+
+def test_force_replay_exports_reference_files(tmp_path):
+    sc = S.reference_scene(ref_traj(), seed=0, frames=41)
+    print("[gpu] force replay", flush=True)
+    sim = Simulator(sc, ckpt_every=10)
+    out = force_calculation_and_animation(sim, 40 * sc.cfg.dt, out_dir=str(tmp_path), save_point_clouds=True)
+    fx = np.load(os.path.join(tmp_path, "force_profile_x.npy"))
+    fz = np.load(os.path.join(tmp_path, "force_profile_z.npy"))
+    sp = np.load(os.path.join(tmp_path, "stick_positions.npy"))
+    assert fx.dtype == np.float32 and fx.shape == (41,) and fz.shape == (41,)
+    assert fx[0] == 0 and fz[0] == 0 and fx[-1] == 0 and fz[-1] == 0
+    assert sp.shape == (39, 3)
+    # the stick moves rigidly with the trajectory (sf:266-269)
+    np.testing.assert_allclose(sp - sp[0], sc.traj[:39] - sc.traj[0], atol=2e-6)
+    assert out["liquid_point_clouds"].shape == (40, sc.n_liquid, 3)
+    o = Oracle("f64")
+    Fo = o.rollout(sc, nsteps=39)
+    assert force_err(np.stack([fx[:40], np.zeros(40), fz[:40]], 1), np.stack([Fo[:, 0], 0 * Fo[:, 0], Fo[:, 2]], 1)) <= 1e-4
+    sim.close()
+
+
+def test_autograd_function_matches_engine_backward():
+    torch = pytest.importorskip("torch")
+    from diffstir_amd.autograd import StirRollout
+    scenes = [S.tiny_scene(16, 15, seed=s) for s in (0, 1)]
+    print("[gpu] autograd", flush=True)
+    with Engine(scenes, ckpt_every=5) as e:
+        p = torch.tensor(np.stack([s.params for s in scenes]), dtype=torch.float64, requires_grad=True)
+        F = StirRollout.apply(p, e)
+        tgt = torch.zeros_like(F)
+        loss = ((F[:, :-1, 0] - tgt[:, :-1, 0]) ** 2).sum() + ((F[:, :-1, 2] - tgt[:, :-1, 2]) ** 2).sum()
+        loss.backward()
+        Fn = F.detach().numpy()
+        g = np.zeros_like(Fn)
+        g[:, :-1, 0] = 2 * Fn[:, :-1, 0]
+        g[:, :-1, 2] = 2 * Fn[:, :-1, 2]
+        e.set_params(p.detach().numpy())
+        e.reset()
+        e.forward()
+        d = e.backward(g)
+    np.testing.assert_allclose(p.grad.numpy(), d, rtol=1e-6)
+
+
+def test_batched_identification_reduces_the_loss():
+    truth = S.hb_params_draw(np.random.default_rng(5))
+    base = S.tiny_scene(16, 20, seed=0, params=truth)
+    print("[gpu] identification", flush=True)
+    from diffstir_amd.identify import synthetic_target
+    tgt = synthetic_target(base, truth, noise=0.01, ckpt_every=5)
+    rng = np.random.default_rng(9)
+    inits = np.stack([np.r_[truth[:3] * np.exp(rng.normal(0, 0.3, 3)), truth[3]] for _ in range(4)])
+    res = identify([base] * 4, tgt, inits, iterations=12, lr=5e-2, ckpt_every=5)
+    L = res["losses"]
+    assert L.shape == (12, 4)
+    assert np.all(L[-1] < L[0])
