@@ -34,6 +34,16 @@ from diffstir_amd.loss import batch_loss  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FWD_BYTES = 192.0            # FP32 x, v, C, F read once and written once per particle-step
 
+# Algorithmic HBM bytes per particle (all species) of one launch of each tiled kernel: the
+# FP32 particle data the kernel must read and write.  P2G and G2P together cover the 192 B
+# forward particle-step (x is read by both, hence 204).
+KERNEL_BYTES_PER_PARTICLE = {
+    "k_p2g_tiled": 96.0 + 36.0,          # read x, v, C, F; write F
+    "k_g2p_tiled": 12.0 + 60.0,          # read x; write x, v, C
+    "k_g2p_adj_tiled": 12.0 + 72.0 + 12.0,          # read x and the adjoints of x, v, C; write dL/dx part
+    "k_p2g_adj_tiled": 96.0 + 36.0 + 36.0 + 12.0 + 96.0,  # read x, v, C, F, affine, adjoint F, dL/dx part; write adjoint
+}
+
 
 def fwd_adj_bytes(K: int) -> float:
     """Algorithmic bytes per fwd+adjoint particle-step with checkpoints every K substeps:
@@ -69,13 +79,14 @@ def grad_eval(eng: Engine, targets, kind):
     return losses, d
 
 
-def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled"):
+def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10):
     scenes = S.workload(workload)
     n_liq = sum(s.n_liquid for s in scenes)
+    n_all = sum(s.n_liquid + s.n_stick + s.n_container for s in scenes)
     T = scenes[0].max_steps
     log(f"[bench] workload {workload}: {len(scenes)} instance(s), {n_liq} liquid particles, {T - 1} substeps, "
         f"checkpoint every {ckpt}")
-    eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels)
+    eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels, kernel_timing_every=timing_every if kernels == "tiled" else 0)
     targets = make_targets(eng, scenes)
     eng.set_params(candidate_params(scenes))
     kind = scenes[0].cfg.loss_kind
@@ -83,6 +94,7 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
         grad_eval(eng, targets, kind)
         log(f"[bench] warmup {i + 1}/{warmup} done")
     eng.synchronize()
+    eng.kernel_timing(reset=True)
     fwd_ms = bwd_ms = 0.0
     t0 = time.perf_counter()
     for i in range(steps):
@@ -96,8 +108,8 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
     if not np.all(np.isfinite(d)):
         raise RuntimeError("non-finite gradient")
     psteps = float(n_liq) * (T - 1) * steps
-    res = dict(scenes=scenes, n_liq=n_liq, T=T, elapsed=elapsed, psteps=psteps, fwd_ms=fwd_ms / steps,
-               bwd_ms=bwd_ms / steps, B=len(scenes))
+    res = dict(scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps, fwd_ms=fwd_ms / steps,
+               bwd_ms=bwd_ms / steps, B=len(scenes), kernels=eng.kernel_timing())
     eng.close()
     return res
 
@@ -119,6 +131,41 @@ def cpu_baseline(workload: str, sample_steps: int, ckpt: int):
                        f"(plus one extra forward), FP64 oracle, {el:.1f} s")
 
 
+def roofline(r) -> dict:
+    """The dominant tiled kernel (largest sampled GPU time): its algorithmic bytes per launch
+    divided by its average launch time, both from this run."""
+    kt = {k: v for k, v in r["kernels"].items() if k in KERNEL_BYTES_PER_PARTICLE and v[1] > 0}
+    if not kt:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None}
+    name = max(kt, key=lambda k: kt[k][0])
+    tot_ms, n = kt[name]
+    avg_s = tot_ms / n / 1e3
+    per_launch = KERNEL_BYTES_PER_PARTICLE[name] * r["n_all"]
+    achieved = per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+            "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
+            "sampled_launches": n,
+            "kernel_avg_us": {k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in r["kernels"].items()}}
+
+
+def calibrate_copy(mib: int):
+    """A plain device-to-device copy of `mib` MiB (three times), for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of a profiled run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,7 +176,13 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels", default="tiled", choices=["tiled", "per_particle"])
+    ap.add_argument("--batch-workload", default="batch32_n64")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0)
     a = ap.parse_args()
+    if a.calibrate_copy_mib:
+        calibrate_copy(a.calibrate_copy_mib)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the instances of a batch share one device "
               "and nothing here uses a second GPU", file=sys.stderr)
@@ -156,11 +209,19 @@ def main():
                    "substeps": r["T"] - 1, "n_grid": r["scenes"][0].cfg.n_grid, "ckpt_every": a.ckpt,
                    "kernels": a.kernels},
         "extra": {"gradient_evals_per_s": r["B"] * a.steps / r["elapsed"], "fwd_ms": r["fwd_ms"],
-                  "adjoint_ms": r["bwd_ms"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "basis": f"whole evaluation, {bpps:.1f} algorithmic B per fwd+adjoint particle-step"},
+                  "adjoint_ms": r["bwd_ms"],
+                  "whole_evaluation_GBps": achieved,
+                  "whole_evaluation_basis": f"{bpps:.1f} algorithmic B per fwd+adjoint particle-step"},
+        "roofline": roofline(r),
     }
+    if not a.no_batch:
+        rb = run_gpu(a.batch_workload, 1, 1, a.ckpt, a.kernels)
+        line["extra"]["batched"] = {
+            "workload": a.batch_workload, "instances": rb["B"], "liquid_particles": rb["n_liq"],
+            "gradient_evals_per_s": rb["B"] / rb["elapsed"],
+            "fwd_adjoint_particle_steps_per_s": rb["psteps"] / rb["elapsed"],
+            "ratio_to_single_instance_evals": (rb["B"] / rb["elapsed"]) / line["extra"]["gradient_evals_per_s"],
+        }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_sample_steps, a.ckpt)
     print(json.dumps(line), flush=True)

@@ -99,6 +99,14 @@ struct ds_ctx {
     long maxch = 0;
     std::vector<int> nch_h;       // chunks per checkpoint
 
+    // sampled per-kernel timing
+    int kt_every = 0;
+    bool kt_on = false;                                    // sampling the current substep
+    std::vector<hipEvent_t> kt_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kt_pending;
+    double kt_ms[DS_NKERNELS] = {0};
+    int64_t kt_n[DS_NKERNELS] = {0};
+
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double fwd_ms = 0.0, bwd_ms = 0.0;
@@ -218,6 +226,55 @@ int check_flag(ds_ctx* c, const char* where)
     return DS_OK;
 }
 
+// ---------------------------------------------------------------------------- kernel timing
+
+hipEvent_t kt_event(ds_ctx* c)
+{
+    if (!c->kt_pool.empty()) {
+        hipEvent_t e = c->kt_pool.back();
+        c->kt_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Bracket a launch with events when the current substep is sampled.
+struct KernelTimer {
+    ds_ctx* c;
+    int id;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(ds_ctx* c_, int id_) : c(c_), id(id_)
+    {
+        if (!c->kt_on) return;
+        a = kt_event(c);
+        b = kt_event(c);
+        if (a && b) (void)hipEventRecord(a, c->stream);
+    }
+    ~KernelTimer()
+    {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->kt_pending.push_back({id, {a, b}});
+    }
+};
+
+// After a synchronisation: add the elapsed time of every recorded pair.
+void kt_collect(ds_ctx* c)
+{
+    for (auto& pr : c->kt_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pr.second.first, pr.second.second) == hipSuccess) {
+            c->kt_ms[pr.first] += ms;
+            c->kt_n[pr.first] += 1;
+        }
+        c->kt_pool.push_back(pr.second.first);
+        c->kt_pool.push_back(pr.second.second);
+    }
+    c->kt_pending.clear();
+}
+
 void free_mode(ds_ctx* c)
 {
     dfree(c->d_seg);
@@ -254,6 +311,13 @@ void free_mode(ds_ctx* c)
 
 void free_all(ds_ctx* c)
 {
+    for (auto& pr : c->kt_pending) {
+        (void)hipEventDestroy(pr.second.first);
+        (void)hipEventDestroy(pr.second.second);
+    }
+    c->kt_pending.clear();
+    for (hipEvent_t e : c->kt_pool) (void)hipEventDestroy(e);
+    c->kt_pool.clear();
     free_mode(c);
     dfree(c->d_meta);
     dfree(c->d_groups);
@@ -455,11 +519,14 @@ void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* p
     const int nch = c->nch_h[cp];
     const int* bnch = c->d_bnch + (size_t)cp * c->NBK;
     const int* cstart = c->d_cstart + (size_t)cp * c->NBK;
-    if (nch > 0)
+    if (nch > 0) {
+        KernelTimer kt(c, DS_K_P2G);
         DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
                         c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, c->d_staging, c->d_pfb, c->dsc, c->P,
                         c->d_flag);
+    }
     const int nb3 = c->NB * c->NB * c->NB;
+    KernelTimer kt(c, DS_K_GRID_REDUCE);
     hipLaunchKernelGGL(k_grid_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart, c->d_pfb,
                        pout, c->d_vgrid, c->dsc, c->NB);
 }
@@ -468,6 +535,7 @@ void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool recor
 {
     const int nch = c->nch_h[cp];
     if (nch == 0) return;
+    KernelTimer kt(c, DS_K_G2P);
     DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_tiled, dim3(nch), dim3(WG), 0, c->stream, in, out,
                     c->d_chunks + (size_t)cp * c->maxch, nch, c->d_vgrid, c->d_traj, c->T, f,
                     record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
@@ -480,8 +548,12 @@ int forward_tiled(ds_ctx* c, int nsteps)
         const int s = c->step;
         const int cp = s / c->K;
         const float* in = c->d_state;
+        c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
         if (s % c->K == 0) {
-            RCHK(sort_into_checkpoint(c, cp));
+            {
+                KernelTimer kt(c, DS_K_SORT);
+                RCHK(sort_into_checkpoint(c, cp));
+            }
             in = c->d_ckpt + (size_t)cp * SF;
         }
         p2g_reduce_tiled(c, in, c->d_state, cp, nullptr, nullptr);
@@ -512,18 +584,26 @@ int backward_tiled(ds_ctx* c)
         // recompute the segment, keeping each step's state and momentum/mass grid
         for (int k = 0; k < L; k++) {
             const int s = s0 + k;
+            c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
             p2g_reduce_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k), aff_of(k));
             if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, s, false);
         }
         // adjoint, last step first
         for (int k = L - 1; k >= 0; k--) {
             const int s = s0 + k;
-            if (nch > 0)
+            c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
+            if (nch > 0) {
+                KernelTimer kt(c, DS_K_G2P_ADJ);
                 DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
                                 chunks, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb, c->d_xbar,
                                 c->dsc, c->P);
-            hipLaunchKernelGGL(k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
-                               c->d_vbfb, grid_of(k), c->d_pbar2, c->dsc, c->NB);
+            }
+            {
+                KernelTimer kt(c, DS_K_VBAR_REDUCE);
+                hipLaunchKernelGGL(k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
+                                   c->d_vbfb, grid_of(k), c->d_pbar2, c->dsc, c->NB);
+            }
+            KernelTimer kt(c, DS_K_P2G_ADJ);
             if (nch > 0)
                 DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), aff_of(k),
                                 c->d_adj, c->d_xbar, chunks, nch, c->d_params, c->d_pbar2, c->d_thbar, c->dsc, c->P);
@@ -686,6 +766,11 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         c->mode = value;
         return DS_OK;
     }
+    if (option == DS_OPT_KERNEL_TIMING) {
+        if (value < 0) return fail(c, DS_EINVAL, "DS_OPT_KERNEL_TIMING takes N >= 0");
+        c->kt_every = value;
+        return DS_OK;
+    }
     if (option == DS_OPT_PARTICLES_PER_THREAD) {
         if (value != 1 && value != 2 && value != 4)
             return fail(c, DS_EINVAL, "DS_OPT_PARTICLES_PER_THREAD takes 1, 2 or 4");
@@ -774,6 +859,8 @@ int ds_forward(ds_ctx* c, int32_t nsteps)
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->fwd_ms = ms;
+    c->kt_on = false;
+    kt_collect(c);
     return check_flag(c, "ds_forward");
 }
 
@@ -819,6 +906,8 @@ int ds_backward(ds_ctx* c, const double* dl_dforces, double* dl_dparams)
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->bwd_ms = ms;
+    c->kt_on = false;
+    kt_collect(c);
     return check_flag(c, "ds_backward");
 }
 
@@ -855,6 +944,20 @@ int ds_last_timing(const ds_ctx* c, double* fwd_ms, double* bwd_ms)
     if (!c) return DS_EINVAL;
     if (fwd_ms) *fwd_ms = c->fwd_ms;
     if (bwd_ms) *bwd_ms = c->bwd_ms;
+    return DS_OK;
+}
+
+int ds_kernel_timing(ds_ctx* c, double* total_ms, int64_t* launches, int32_t reset)
+{
+    if (!c) return DS_EINVAL;
+    for (int k = 0; k < DS_NKERNELS; k++) {
+        if (total_ms) total_ms[k] = c->kt_ms[k];
+        if (launches) launches[k] = c->kt_n[k];
+        if (reset) {
+            c->kt_ms[k] = 0.0;
+            c->kt_n[k] = 0;
+        }
+    }
     return DS_OK;
 }
 

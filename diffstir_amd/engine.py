@@ -45,7 +45,7 @@ class Engine:
     KERNELS = {"tiled": 0, "per_particle": 1}
 
     def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None,
-                 kernels: str = "tiled", particles_per_thread: int = 2):
+                 kernels: str = "tiled", particles_per_thread: int = 2, kernel_timing_every: int = 0):
         self.lib = _lib.load()
         scenes = list(scenes)
         if not scenes:
@@ -72,6 +72,7 @@ class Engine:
         self.kernels = kernels
         self._check(self.lib.ds_set_option(self.ctx, 1, self.KERNELS[kernels]))
         self._check(self.lib.ds_set_option(self.ctx, 2, int(particles_per_thread)))
+        self._check(self.lib.ds_set_option(self.ctx, 3, int(kernel_timing_every)))
         for b, s in enumerate(scenes):
             xl = _f32(s.x_liquid, (-1, 3))
             xs = _f32(s.x_stick, (-1, 3))
@@ -146,6 +147,14 @@ class Engine:
         b = C.c_double()
         self._check(self.lib.ds_last_timing(self.ctx, C.byref(f), C.byref(b)))
         return f.value, b.value
+
+    def kernel_timing(self, reset: bool = False) -> dict:
+        """{kernel name: (total ms, launches)} from the sampled HIP-event timing."""
+        n = len(_lib.KERNEL_NAMES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        self._check(self.lib.ds_kernel_timing(self.ctx, ms, cnt, int(bool(reset))))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.KERNEL_NAMES)}
 
     def read_liquid(self, inst: int = 0) -> dict:
         n = self.scenes[inst].n_liquid

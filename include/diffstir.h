@@ -45,6 +45,19 @@ enum { DS_MODEL_CARREAU = 0, DS_MODEL_CROSS = 1, DS_MODEL_HERSCHEL_BULKLEY = 2 }
 enum {
     DS_OPT_KERNELS = 1,   /* DS_KERNELS_TILED (default) or DS_KERNELS_PER_PARTICLE; before ds_reset */
     DS_OPT_PARTICLES_PER_THREAD = 2,   /* tiled kernels: 1, 2 (default) or 4 particles per thread */
+    DS_OPT_KERNEL_TIMING = 3,          /* time every N-th substep's kernels with HIP events (0 = off) */
+};
+
+/* Kernels timed by DS_OPT_KERNEL_TIMING (tiled family). */
+enum {
+    DS_K_P2G = 0,          /* k_p2g_tiled: P2G into staging slots */
+    DS_K_GRID_REDUCE = 1,  /* k_grid_reduce: slots -> grid update -> velocity */
+    DS_K_G2P = 2,          /* k_g2p_tiled */
+    DS_K_G2P_ADJ = 3,      /* k_g2p_adj_tiled */
+    DS_K_VBAR_REDUCE = 4,  /* k_vbar_reduce */
+    DS_K_P2G_ADJ = 5,      /* k_p2g_adj_tiled */
+    DS_K_SORT = 6,         /* the whole bucket sort of one checkpoint */
+    DS_NKERNELS = 7
 };
 enum { DS_KERNELS_TILED = 0, DS_KERNELS_PER_PARTICLE = 1 };
 
@@ -113,6 +126,10 @@ int ds_synchronize(ds_ctx* ctx);
 /* Timing of the most recent ds_forward / ds_backward, measured with HIP events on the
  * context's stream: elapsed milliseconds and kernel launches. */
 int ds_last_timing(const ds_ctx* ctx, double* fwd_ms, double* bwd_ms);
+
+/* Per-kernel GPU time accumulated since the last reset by DS_OPT_KERNEL_TIMING sampling:
+ * total_ms[DS_NKERNELS], launches[DS_NKERNELS]; reset != 0 clears the counters afterwards. */
+int ds_kernel_timing(ds_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);
 
 /* FP64 3x3 signed SVD on the device, for tests: F[n][9] -> U[n][9], S[n][3], V[n][9]
  * with F = U diag(S) V^T, det U = det V = +1 and S[2] carrying the sign of det F.
