@@ -108,7 +108,7 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
     if not np.all(np.isfinite(d)):
         raise RuntimeError("non-finite gradient")
     psteps = float(n_liq) * (T - 1) * steps
-    res = dict(scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps, fwd_ms=fwd_ms / steps,
+    res = dict(workload=workload, scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps, fwd_ms=fwd_ms / steps,
                bwd_ms=bwd_ms / steps, B=len(scenes), kernels=eng.kernel_timing())
     eng.close()
     return res
@@ -142,8 +142,15 @@ def roofline(r) -> dict:
     avg_s = tot_ms / n / 1e3
     per_launch = KERNEL_BYTES_PER_PARTICLE[name] * r["n_all"]
     achieved = per_launch / avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{r['workload']}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("kernel") == name:
+            traffic = t["bytes_per_launch"]
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": name,
             "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
             "sampled_launches": n,
             "kernel_avg_us": {k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in r["kernels"].items()}}
