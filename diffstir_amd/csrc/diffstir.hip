@@ -94,9 +94,16 @@ struct ds_ctx {
     int* d_rank = nullptr;
     int* d_rrpos = nullptr;
     Chunk* d_chunks = nullptr;    // ncp x maxch
+    int* d_rows = nullptr;        // ncp x (P + 1) row table: first slot of every row
+    GChunk* d_gchunks = nullptr;  // ncp x maxgch chunks of the gather-only kernels
+    int* d_bgch = nullptr;        // gather chunks per bucket
+    int* d_gstart = nullptr;      // first gather chunk of each bucket
+    int* d_brows = nullptr;       // rows per bucket
+    int* d_rstart = nullptr;      // first row of each bucket
     int* d_src = nullptr;         // ncp x P
     int NB = 0, NBK = 0;
-    long maxch = 0;
+    long maxch = 0, maxgch = 0;
+    std::vector<int> ngch_h;      // gather chunks per checkpoint
     std::vector<int> nch_h;       // chunks per checkpoint
 
     // sampled per-kernel timing
@@ -305,6 +312,12 @@ void free_mode(ds_ctx* c)
     dfree(c->d_rank);
     dfree(c->d_rrpos);
     dfree(c->d_chunks);
+    dfree(c->d_rows);
+    dfree(c->d_gchunks);
+    dfree(c->d_bgch);
+    dfree(c->d_gstart);
+    dfree(c->d_brows);
+    dfree(c->d_rstart);
     dfree(c->d_src);
     c->alloc_mode = -1;
 }
@@ -362,7 +375,9 @@ int alloc_mode(ds_ctx* c)
     } else {
         c->NB = (c->desc.n_grid + TB - 1) / TB;
         c->NBK = c->B * 3 * c->NB * c->NB * c->NB;
-        c->maxch = (long)c->P / 64 + std::min<long>(c->P, c->NBK) + 1;
+        // chunks of a bucket: ceil(rows / (WAVES * PPT)) <= rows / WAVES + 1, and rows <= particles
+        c->maxch = (long)c->P / WAVES + std::min<long>(c->P, c->NBK) + 1;
+        c->maxgch = (long)c->P / WG + std::min<long>(c->P, c->NBK) + 1;
         MALLOC(c->d_vgrid, G);
         MALLOC(c->d_pbar2, G);
         MALLOC(c->d_pfb, G);
@@ -379,13 +394,20 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_bnch, (size_t)c->NBK * c->ncp);
         MALLOC(c->d_bstart, c->NBK);
         MALLOC(c->d_cstart, (size_t)c->NBK * c->ncp);
-        MALLOC(c->d_tot, 2);
+        MALLOC(c->d_tot, 4);
+        MALLOC(c->d_bgch, c->NBK);
+        MALLOC(c->d_gstart, c->NBK);
+        MALLOC(c->d_gchunks, (size_t)c->maxgch * c->ncp);
+        MALLOC(c->d_brows, c->NBK);
+        MALLOC(c->d_rstart, c->NBK);
+        MALLOC(c->d_rows, (size_t)(c->P + 1) * c->ncp);
         MALLOC(c->d_key, c->P);
         MALLOC(c->d_rank, c->P);
         MALLOC(c->d_rrpos, c->P);
         MALLOC(c->d_chunks, (size_t)c->maxch * c->ncp);
         MALLOC(c->d_src, (size_t)c->P * c->ncp);
         c->nch_h.assign(c->ncp, 0);
+        c->ngch_h.assign(c->ncp, 0);
         fill_zero_f4(c, c->d_pfb, G);
         fill_zero_f4(c, c->d_vbfb, G);
     }
@@ -474,27 +496,32 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     fill_i32(c, c->d_cnt, (long)c->NBK * 64, 0);
     hipLaunchKernelGGL(k_sort_count, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_meta,
                        c->d_groups, c->d_key, c->d_rank, c->d_cnt, c->dsc, c->NB, c->P);
-    const int chunk = WG * c->ppt;
+    const int rpc = WAVES * c->ppt;   // rows per chunk
+    const int gch = WG * c->ppt;      // particles per gather chunk
     c->ppt_sorted = c->ppt;
+    int* rows = c->d_rows + (size_t)cp * (c->P + 1);
+    GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
     hipLaunchKernelGGL(k_bucket_totals, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bcount, bnch, c->NBK, chunk);
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, c->stream, c->d_bcount, bnch, c->d_bstart, cstart, c->NBK,
-                       c->d_tot);
+                       c->d_celloff, c->d_bcount, c->d_brows, bnch, c->d_bgch, c->NBK, rpc, gch);
+    Scan4 q{{c->d_bcount, bnch, c->d_brows, c->d_bgch}, {c->d_bstart, cstart, c->d_rstart, c->d_gstart}};
+    hipLaunchKernelGGL(k_scan4, dim3(1), dim3(1024), 0, c->stream, q, c->NBK, c->d_tot);
     hipLaunchKernelGGL(k_bucket_layout, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bstart, cstart, c->d_rrpos, chunks, c->NB, c->NBK, chunk);
+                       c->d_celloff, c->d_bstart, c->d_rstart, cstart, c->d_gstart, c->d_rrpos, rows, chunks, gchunks,
+                       c->NB, c->NBK, rpc, gch);
     fill_i32(c, c->d_meta2, c->P, -1);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
                        c->d_rank, c->d_celloff, c->d_bstart, c->d_rrpos, c->d_meta, c->d_meta2, c->d_pid, c->d_pid2,
                        src, c->P);
     std::swap(c->d_meta, c->d_meta2);
     std::swap(c->d_pid, c->d_pid2);
-    int tot[2] = {0, 0};
+    int tot[4] = {0, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(tot, c->d_tot, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (tot[0] != c->nreal || tot[1] < 0 || tot[1] > c->maxch)
+    if (tot[0] != c->nreal || tot[1] < 0 || tot[1] > c->maxch || tot[2] > c->nreal || tot[3] > c->maxgch)
         return fail(c, DS_EHIP, "bucket sort lost particles (" + std::to_string(tot[0]) + " of " +
                                     std::to_string(c->nreal) + ")");
     c->nch_h[cp] = tot[1];
+    c->ngch_h[cp] = tot[3];
     if (tot[1] > c->staging_cap) {
         const long cap = (long)tot[1] + tot[1] / 4 + 16;
         dfree(c->d_staging);
@@ -522,7 +549,8 @@ void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* p
     if (nch > 0) {
         KernelTimer kt(c, DS_K_P2G);
         DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
-                        c->d_chunks + (size_t)cp * c->maxch, nch, c->d_params, c->d_staging, c->d_pfb, c->dsc, c->P,
+                        c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1), nch, c->d_params,
+                        c->d_staging, c->d_pfb, c->dsc, c->P,
                         c->d_flag);
     }
     const int nb3 = c->NB * c->NB * c->NB;
@@ -533,11 +561,11 @@ void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* p
 
 void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool record)
 {
-    const int nch = c->nch_h[cp];
-    if (nch == 0) return;
+    const int ngc = c->ngch_h[cp];
+    if (ngc == 0) return;
     KernelTimer kt(c, DS_K_G2P);
-    DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_tiled, dim3(nch), dim3(WG), 0, c->stream, in, out,
-                    c->d_chunks + (size_t)cp * c->maxch, nch, c->d_vgrid, c->d_traj, c->T, f,
+    DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_tiled, dim3(ngc), dim3(WG), 0, c->stream, in, out,
+                    c->d_gchunks + (size_t)cp * c->maxgch, ngc, c->d_vgrid, c->d_traj, c->T, f,
                     record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
 }
 
@@ -575,6 +603,9 @@ int backward_tiled(ds_ctx* c)
         const int L = s1 - s0;
         const int nch = c->nch_h[cp];
         const Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
+        const int* rows = c->d_rows + (size_t)cp * (c->P + 1);
+        const int ngc = c->ngch_h[cp];
+        const GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
         const int* bnch = c->d_bnch + (size_t)cp * c->NBK;
         const int* cstart = c->d_cstart + (size_t)cp * c->NBK;
         const float* ck = c->d_ckpt + (size_t)cp * SF;
@@ -595,7 +626,7 @@ int backward_tiled(ds_ctx* c)
             if (nch > 0) {
                 KernelTimer kt(c, DS_K_G2P_ADJ);
                 DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
-                                chunks, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb, c->d_xbar,
+                                chunks, rows, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb, c->d_xbar,
                                 c->dsc, c->P);
             }
             {
@@ -604,9 +635,9 @@ int backward_tiled(ds_ctx* c)
                                    c->d_vbfb, grid_of(k), c->d_pbar2, c->dsc, c->NB);
             }
             KernelTimer kt(c, DS_K_P2G_ADJ);
-            if (nch > 0)
-                DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), aff_of(k),
-                                c->d_adj, c->d_xbar, chunks, nch, c->d_params, c->d_pbar2, c->d_thbar, c->dsc, c->P);
+            if (ngc > 0)
+                DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(ngc), dim3(WG), 0, c->stream, st_of(k), aff_of(k),
+                                c->d_adj, c->d_xbar, gchunks, ngc, c->d_params, c->d_pbar2, c->d_thbar, c->dsc, c->P);
         }
         if (cp > 0) {
             hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
@@ -979,6 +1010,27 @@ int ds_svd3(const double* F, int32_t n, double* U, double* S, double* V)
     if (e == hipSuccess) e = hipMemcpy(U, dU, nb9, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(S, dS, nb3, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(V, dV, nb9, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(nullptr, DS_EHIP, hipGetErrorString(e));
+    return DS_OK;
+}
+
+int ds_svd3_values(const double* F, int32_t n, double* S)
+{
+    if (!F || !S || n < 1) return fail(nullptr, DS_EINVAL, "bad argument");
+    const size_t nb9 = 9 * sizeof(double) * n, nb3 = 3 * sizeof(double) * n;
+    double* buf;
+    if (hipMalloc((void**)&buf, nb9 + nb3) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(nullptr, DS_ENOMEM, "hipMalloc");
+    }
+    double *dF = buf, *dS = dF + 9 * (size_t)n;
+    hipError_t e = hipMemcpy(dF, F, nb9, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_svd3_values_test, dim3(nblocks(n)), dim3(256), 0, 0, dF, n, dS);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(S, dS, nb3, hipMemcpyDeviceToHost);
     (void)hipFree(buf);
     if (e != hipSuccess) return fail(nullptr, DS_EHIP, hipGetErrorString(e));
     return DS_OK;

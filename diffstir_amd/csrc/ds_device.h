@@ -139,36 +139,80 @@ __device__ __forceinline__ Visc viscosity(int model, const float th[4], float g)
 
 // ---------------------------------------------------------------------------- FP64 SVD
 
-// One cyclic-Jacobi rotation on the symmetric 3x3 A (stored a00,a11,a22,a01,a02,a12),
-// annihilating A[p][q] and accumulating the rotation into the columns of V
-// (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.5.2).
-template <int P, int Q>
+// FP64 reciprocal and reciprocal square root: the hardware estimates (v_rcp_f64, v_rsq_f64)
+// refined by two Newton-Raphson steps each.  Every step doubles the number of correct bits,
+// so any estimate good to 2^-14 ends within an ulp or two of the exact value.  For positive,
+// normal, finite arguments only (no IEEE special-case handling): the SVD below keeps its
+// arguments there or discards the result.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsqrt64(double a)
+{
+    double y = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    double t = fma(-h * y, y, 0.5);     // y <- y (3 - a y^2) / 2
+    y = fma(y, t, y);
+    t = fma(-h * y, y, 0.5);
+    return fma(y, t, y);
+}
+
+// One cyclic-Jacobi rotation on the symmetric 3x3 A, annihilating A[p][q] and (kV)
+// accumulating the rotation into the columns of V (Golub & Van Loan, Matrix Computations,
+// 4th ed., sec. 8.5.2).  With d = a_qq - a_pp and e = 2 a_pq, the tangent of the rotation
+// angle is the smaller root of t^2 + 2 (d/e) t - 1 = 0, t = sgn(d) e / (|d| + sqrt(d^2 + e^2)),
+// written without the quotient d/e; c = 1/sqrt(1 + t^2), s = t c.
+template <int P, int Q, bool kV>
 __device__ __forceinline__ void jacobi_rot(double A[3][3], double V[3][3])
 {
     constexpr int R = 3 - P - Q;
-    double apq = A[P][Q];
-    if (apq == 0.0) return;
-    double theta = (A[Q][Q] - A[P][P]) / (2.0 * apq);
-    double t;
-    if (fabs(theta) > 1e150) {
-        t = 0.5 / theta;
-    } else {
-        t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
-        t = theta < 0.0 ? -t : t;
-    }
-    double c = 1.0 / sqrt(t * t + 1.0);
-    double s = t * c;
+    const double apq = A[P][Q];
+    const double d = A[Q][Q] - A[P][P];
+    const double e = 2.0 * apq;
+    const double r2 = fma(d, d, e * e);
+    const double den = fabs(d) + r2 * rsqrt64(r2);
+    const double t = r2 > 0.0 ? (d >= 0.0 ? e : -e) * rcp64(den) : 0.0;
+    const double c = rsqrt64(fma(t, t, 1.0));
+    const double s = t * c;
     A[P][P] -= t * apq;
     A[Q][Q] += t * apq;
     A[P][Q] = A[Q][P] = 0.0;
-    double arp = A[R][P], arq = A[R][Q];
+    const double arp = A[R][P], arq = A[R][Q];
     A[R][P] = A[P][R] = c * arp - s * arq;
     A[R][Q] = A[Q][R] = s * arp + c * arq;
+    if (kV) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        double vp = V[i][P], vq = V[i][Q];
-        V[i][P] = c * vp - s * vq;
-        V[i][Q] = s * vp + c * vq;
+        for (int i = 0; i < 3; i++) {
+            const double vp = V[i][P], vq = V[i][Q];
+            V[i][P] = c * vp - s * vq;
+            V[i][Q] = s * vp + c * vq;
+        }
+    }
+}
+
+// A = F^T F diagonalised by cyclic Jacobi sweeps (at most 8; stop once the off-diagonal mass is
+// below 1e-32 of the diagonal's, i.e. relative 1e-16).
+template <bool kV>
+__device__ __forceinline__ void jacobi_eig_FtF(const double F[9], double A[3][3], double V[3][3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            A[i][j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
+    for (int sweep = 0; sweep < 8; sweep++) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-32 * dia) break;
+        jacobi_rot<0, 1, kV>(A, V);
+        jacobi_rot<0, 2, kV>(A, V);
+        jacobi_rot<1, 2, kV>(A, V);
     }
 }
 
@@ -192,20 +236,8 @@ __device__ __forceinline__ void swap_cols(double V[3][3], double S[3], int a, in
 __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3], double Vo[9])
 {
     double A[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            A[i][j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
     double V[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
-    for (int sweep = 0; sweep < 8; sweep++) {
-        double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
-        double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
-        if (off <= 1e-32 * dia) break;
-        jacobi_rot<0, 1>(A, V);
-        jacobi_rot<0, 2>(A, V);
-        jacobi_rot<1, 2>(A, V);
-    }
+    jacobi_eig_FtF<true>(F, A, V);
     double L[3] = {A[0][0], A[1][1], A[2][2]};
     if (L[0] < L[1]) swap_cols(V, L, 0, 1);
     if (L[0] < L[2]) swap_cols(V, L, 0, 2);
@@ -218,24 +250,28 @@ __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3]
         for (int i = 0; i < 3; i++)
             b[c][i] = F[i * 3 + 0] * V[0][c] + F[i * 3 + 1] * V[1][c] + F[i * 3 + 2] * V[2][c];
     double u0[3], u1[3], u2[3];
-    double n0 = sqrt(b[0][0] * b[0][0] + b[0][1] * b[0][1] + b[0][2] * b[0][2]);
-    if (n0 > 1e-300) {
-        double in = 1.0 / n0;
+    const double nn0 = b[0][0] * b[0][0] + b[0][1] * b[0][1] + b[0][2] * b[0][2];
+    double n0 = 0.0;
+    if (nn0 > 1e-280) {
+        const double in = rsqrt64(nn0);
+        n0 = nn0 * in;
         u0[0] = b[0][0] * in; u0[1] = b[0][1] * in; u0[2] = b[0][2] * in;
     } else {
         u0[0] = 1.0; u0[1] = 0.0; u0[2] = 0.0;
     }
-    double d01 = u0[0] * b[1][0] + u0[1] * b[1][1] + u0[2] * b[1][2];
-    double r1[3] = {b[1][0] - d01 * u0[0], b[1][1] - d01 * u0[1], b[1][2] - d01 * u0[2]};
-    double n1 = sqrt(r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
-    if (n1 > 1e-300) {
-        double in = 1.0 / n1;
+    const double d01 = u0[0] * b[1][0] + u0[1] * b[1][1] + u0[2] * b[1][2];
+    const double r1[3] = {b[1][0] - d01 * u0[0], b[1][1] - d01 * u0[1], b[1][2] - d01 * u0[2]};
+    const double nn1 = r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2];
+    double n1 = 0.0;
+    if (nn1 > 1e-280) {
+        const double in = rsqrt64(nn1);
+        n1 = nn1 * in;
         u1[0] = r1[0] * in; u1[1] = r1[1] * in; u1[2] = r1[2] * in;
     } else {
         // any unit vector orthogonal to u0
-        double ax = fabs(u0[0]) < 0.9 ? 1.0 : 0.0, ay = 1.0 - ax;
-        double t[3] = {u0[1] * 0.0 - u0[2] * ay, u0[2] * ax - u0[0] * 0.0, u0[0] * ay - u0[1] * ax};
-        double nt = 1.0 / sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        const double ax = fabs(u0[0]) < 0.9 ? 1.0 : 0.0, ay = 1.0 - ax;
+        const double t[3] = {-u0[2] * ay, u0[2] * ax, u0[0] * ay - u0[1] * ax};
+        const double nt = rsqrt64(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
         u1[0] = t[0] * nt; u1[1] = t[1] * nt; u1[2] = t[2] * nt;
     }
     u2[0] = u0[1] * u1[2] - u0[2] * u1[1];
@@ -253,6 +289,25 @@ __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3]
         Vo[i * 3 + 1] = V[i][1];
         Vo[i * 3 + 2] = V[i][2];
     }
+}
+
+// The singular values alone, in the same signed convention as svd3: S = sqrt of the
+// eigenvalues of F^T F from the same Jacobi sweeps (no V accumulated), sorted descending, with
+// the sign of det F on the smallest.  S[0]*S[1]*S[2] = det F up to rounding.
+__device__ __forceinline__ void svd3_values(const double F[9], double S[3])
+{
+    double A[3][3], V[3][3];
+    jacobi_eig_FtF<false>(F, A, V);
+    const double l0 = fmax(A[0][0], 0.0), l1 = fmax(A[1][1], 0.0), l2 = fmax(A[2][2], 0.0);
+    const double hi = fmax(fmax(l0, l1), l2);
+    const double lo = fmin(fmin(l0, l1), l2);
+    const double mid = fmax(fmin(l0, l1), fmin(fmax(l0, l1), l2));
+    const double detF = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6])
+                      + F[2] * (F[3] * F[7] - F[4] * F[6]);
+    S[0] = hi > 0.0 ? hi * rsqrt64(hi) : 0.0;
+    S[1] = mid > 0.0 ? mid * rsqrt64(mid) : 0.0;
+    const double s2 = lo > 0.0 ? lo * rsqrt64(lo) : 0.0;
+    S[2] = detF < 0.0 ? -s2 : s2;
 }
 
 // clamp of simulation_functions.py:61-67
@@ -338,6 +393,7 @@ struct LiquidP2G {
     double U[9], S[3], V[9];
 };
 
+template <bool kFull>
 __device__ __forceinline__ void liquid_p2g(const DevScene& sc, const float th[4], double kb, const float C[9],
                                            const float F[9], LiquidP2G& q)
 {
@@ -353,7 +409,10 @@ __device__ __forceinline__ void liquid_p2g(const DevScene& sc, const float th[4]
     double Fd[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) Fd[i] = (double)q.Fn[i];
-    svd3(Fd, q.U, q.S, q.V);
+    if (kFull)
+        svd3(Fd, q.U, q.S, q.V);
+    else
+        svd3_values(Fd, q.S);
     double J = q.S[0] * q.S[1] * q.S[2];
     q.J = J;
     float n2 = 0.0f;

@@ -56,7 +56,7 @@ __device__ __forceinline__ void p2g_prepare(const DevScene& sc, int sp, const In
     pl.m = species_mass(sc, sp);
     if (sp == LIQUID) {
         LiquidP2G L;
-        liquid_p2g(sc, ip.th, ip.kb, q.C, q.F, L);
+        liquid_p2g<false>(sc, ip.th, ip.kb, q.C, q.F, L);
 #pragma unroll
         for (int a = 0; a < 9; a++) {
             Fn[a] = L.Fn[a];
@@ -170,28 +170,33 @@ __device__ __forceinline__ void g2p_adj_scatter(const DevScene& sc, const Stenci
 }
 
 // Adjoint of G2P for one liquid particle, position half: the weight-gradient part of dL/dx
-// (the weights and offsets of the gather depend on x).
-template <class Gather>
+// (the weights and offsets of the gather depend on x).  kRolled keeps the loop over x-slices
+// rolled: the form for gathers from global memory, whose 27 inlined loads would otherwise
+// all be in flight at once.
+template <bool kRolled = false, class Gather>
 __device__ __forceinline__ void g2p_adj_xgrad(const DevScene& sc, const Stencil& s, const float vt[3],
                                               const float Cb[9], Gather& gv, float xb[3])
 {
     float fxb[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
+    auto slice = [&](int i) {
+        // one x-slice of nodes at a time (see p2g_adj_gather)
+        asm volatile("" ::: "memory");
+        const float w0 = i == 0 ? s.w[0][0] : (i == 1 ? s.w[0][1] : s.w[0][2]);
+        const float dw0 = i == 0 ? s.dw[0][0] : (i == 1 ? s.dw[0][1] : s.dw[0][2]);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
-                const float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
+                const float W = w0 * s.w[1][j] * s.w[2][k];
                 float e[3];
 #pragma unroll
                 for (int a = 0; a < 3; a++)
                     e[a] = vt[a] + sc.c4 * (Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2]);
                 const float3 g = gv.get(s.b[0] + i, s.b[1] + j, s.b[2] + k);
                 const float Wb = g.x * e[0] + g.y * e[1] + g.z * e[2];
-                const float dW[3] = {s.dw[0][i] * s.w[1][j] * s.w[2][k], s.w[0][i] * s.dw[1][j] * s.w[2][k],
-                                     s.w[0][i] * s.w[1][j] * s.dw[2][k]};
+                const float dW[3] = {dw0 * s.w[1][j] * s.w[2][k], w0 * s.dw[1][j] * s.w[2][k],
+                                     w0 * s.w[1][j] * s.dw[2][k]};
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     const float ctg = Cb[0 * 3 + c] * g.x + Cb[1 * 3 + c] * g.y + Cb[2 * 3 + c] * g.z;
@@ -199,6 +204,13 @@ __device__ __forceinline__ void g2p_adj_xgrad(const DevScene& sc, const Stencil&
                 }
             }
         }
+    };
+    if (kRolled) {
+#pragma unroll 1
+        for (int i = 0; i < 3; i++) slice(i);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) slice(i);
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) xb[a] = fxb[a] * sc.inv_dx;
@@ -344,7 +356,7 @@ __device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstPar
                                                double tb[4])
 {
     LiquidP2G L;
-    liquid_p2g(sc, ip.th, ip.kb, q.C, q.F, L);
+    liquid_p2g<true>(sc, ip.th, ip.kb, q.C, q.F, L);
     const float u[3] = {q.v[0], q.v[1] + sc.gdt, q.v[2]};
     float ub[3], Ab[9], fxb[3];
     p2g_adj_gather(sc, s, L.A, u, acc, ub, Ab, fxb);

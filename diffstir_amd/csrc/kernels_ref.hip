@@ -327,6 +327,16 @@ __global__ void k_svd3_test(const double* __restrict__ F, int n, double* __restr
     for (int a = 0; a < 3; a++) S[3 * (long)p + a] = s[a];
 }
 
+__global__ void k_svd3_values_test(const double* __restrict__ F, int n, double* __restrict__ S)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double f[9], s[3];
+    for (int a = 0; a < 9; a++) f[a] = F[9 * (long)p + a];
+    svd3_values(f, s);
+    for (int a = 0; a < 3; a++) S[3 * (long)p + a] = s[a];
+}
+
 __global__ void k_svd3_bwd_test(const double* __restrict__ U, const double* __restrict__ S,
                                 const double* __restrict__ V, const double* __restrict__ gU,
                                 const double* __restrict__ gS, const double* __restrict__ gV, int n,

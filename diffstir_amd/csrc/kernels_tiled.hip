@@ -47,9 +47,29 @@ constexpr int TNODES = TS * TS * TS;  // nodes of a staging slot (dense, index l
 constexpr int WG = 256;               // threads per workgroup
 constexpr int WAVES = WG / 64;
 
+// A chunk: up to WAVES * PPT consecutive rows of one bucket (row r of a bucket holds the r-th
+// particle of every cell with more than r particles, so the lanes of a row have distinct
+// stencil bases).  Rows are global indices into the checkpoint's row table: row k covers
+// slots [rows[k], rows[k + 1]).
 struct Chunk {
+    int inst, species, ox, oy, oz, start, row0, nrows;
+};
+
+// A chunk for the gather-only kernels (G2P, adjoint P2G), which have no scatter conflicts: a
+// contiguous run of up to WG * PPT particles of one bucket.
+struct GChunk {
     int inst, species, ox, oy, oz, start, count, pad;
 };
+
+// The slots of lane `lane` in row `lr` (local to the chunk) of chunk ch: sets p and returns
+// whether the lane has a particle there.  lr is uniform across the wavefront.
+__device__ __forceinline__ bool chunk_slot(const Chunk& ch, const int* __restrict__ rows, int lr, int lane, int& p)
+{
+    if (lr >= ch.nrows) return false;
+    const int rs = rows[ch.row0 + lr], re = rows[ch.row0 + lr + 1];
+    p = rs + lane;
+    return p < re;
+}
 
 __device__ __forceinline__ int tile_id(int lx, int ly, int lz) { return lx * TSX + ly * TSY + lz; }
 
@@ -62,6 +82,22 @@ __device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0
 // Compiler-only memory barrier: keeps LDS accesses in program order without waiting for
 // them (a wavefront's LDS operations execute in order in hardware).
 __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
+// Makes a stencil's offsets and weights opaque to the compiler at this point.  The claim loop
+// below runs the scatter in whichever pass the lane wins; without this, loop-invariant code
+// motion computes all 27 node contributions before the loop and keeps them live across it
+// (over 100 extra registers, halving the wavefronts a SIMD can hold).
+__device__ __forceinline__ Stencil opaque(const Stencil& s)
+{
+    Stencil t = s;
+#pragma unroll
+    for (int a = 0; a < 3; a++) asm volatile("" : "+v"(t.fx[a]));
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) asm volatile("" : "+v"(t.w[a][b]));
+    return t;
+}
 
 // Read-modify-write scatter into a wavefront's private tile (see the claim protocol above).
 // The barrier after each node keeps the next node's reads behind this node's writes: another
@@ -95,50 +131,26 @@ struct WaveScatter3 {
     }
 };
 
-// Gather accessors for a workgroup tile with a per-node fall-back to global memory for
-// stencils that left the tile.
-struct TileGatherV {
+// Gather accessors for a workgroup tile.  A particle whose stencil left the tile gathers from
+// global memory instead; the kernels branch once per particle between the two accessors, so
+// each inlined 27-node gather is a single straight-line form.
+struct TileV {
     const float* t;
     int x0, y0, z0;
-    bool in_tile;
-    GatherVout fb;
     __device__ __forceinline__ float3 get(int i, int j, int k) const
     {
-        if (in_tile) {
-            const int id = tile_id(i - x0, j - y0, k - z0);
-            return make_float3(t[id], t[TILE + id], t[2 * TILE + id]);
-        }
-        return fb.get(i, j, k);
+        const int id = tile_id(i - x0, j - y0, k - z0);
+        return make_float3(t[id], t[TILE + id], t[2 * TILE + id]);
     }
 };
 
-struct TileGatherVfromP {
+struct TileP4 {
     const float* t;
     int x0, y0, z0;
-    bool in_tile;
-    GatherVfromP fb;
-    __device__ __forceinline__ float3 get(int i, int j, int k) const
-    {
-        if (in_tile) {
-            const int id = tile_id(i - x0, j - y0, k - z0);
-            return make_float3(t[id], t[TILE + id], t[2 * TILE + id]);
-        }
-        return fb.get(i, j, k);
-    }
-};
-
-struct TileGatherPbar {
-    const float* t;
-    int x0, y0, z0;
-    bool in_tile;
-    GatherPbar fb;
     __device__ __forceinline__ float4 get(int i, int j, int k) const
     {
-        if (in_tile) {
-            const int id = tile_id(i - x0, j - y0, k - z0);
-            return make_float4(t[id], t[TILE + id], t[2 * TILE + id], t[3 * TILE + id]);
-        }
-        return fb.get(i, j, k);
+        const int id = tile_id(i - x0, j - y0, k - z0);
+        return make_float4(t[id], t[TILE + id], t[2 * TILE + id], t[3 * TILE + id]);
     }
 };
 
@@ -184,18 +196,19 @@ __device__ __forceinline__ void store_staging(const float* wt, int wstride, int 
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
                                                   float* __restrict__ aout, const Chunk* __restrict__ chunks,
+                                                  const int* __restrict__ rows,
                                                   int nchunks,
                                                   const double* __restrict__ params, float4* __restrict__ staging,
                                                   float4* __restrict__ fallback, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
 {
     __shared__ float wt[WAVES * 4 * TILE];
-    __shared__ int own[WAVES * TNODES];
+    __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int* ow = own + wave * TNODES;
+    unsigned char* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         zero_lds(wt, WAVES * 4 * TILE);
         __syncthreads();
@@ -205,13 +218,12 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
         WaveScatter4 acc{wt + wave * 4 * TILE, x0, y0, z0};
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
-            const int idx = r * WG + threadIdx.x;
-            const bool valid = idx < ch.count;
+            int p = 0;
+            const bool valid = chunk_slot(ch, rows, r * WAVES + wave, lane, p);
             Stencil s;
             P2GPayload pl;
             bool it = false;
             if (valid) {
-                const int p = ch.start + idx;
                 ParticleIn q;
                 load_particle_t(in, LP, p, ch.species == LIQUID, q);
                 if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
@@ -235,10 +247,10 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
             const int key = pending ? node_key(s, x0, y0, z0) : 0;
             while (__ballot(pending)) {
                 lds_order();
-                if (pending) ow[key] = lane;
+                if (pending) ow[key] = (unsigned char)lane;
                 lds_order();
                 if (pending && ow[key] == lane) {
-                    p2g_scatter(sc, s, pl, acc);
+                    p2g_scatter(sc, opaque(s), pl, acc);
                     pending = false;
                 }
             }
@@ -335,7 +347,7 @@ __global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ s
 
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
-                                                  const Chunk* __restrict__ chunks, int nchunks,
+                                                  const GChunk* __restrict__ chunks, int nchunks,
                                                   const float4* __restrict__ vgrid, const float* __restrict__ traj,
                                                   int T, int f, double* __restrict__ dvsum, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
@@ -346,7 +358,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
     const long nn = (long)n * n * n;
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const Chunk ch = chunks[c];
+        const GChunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const float4* vi = vgrid + (long)ch.inst * nn;
         for (int e = threadIdx.x; e < TNODES; e += WG) {
@@ -373,8 +385,13 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
             Stencil s;
             if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
             float nv[3], nC[9];
-            TileGatherV acc{tv, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherVout{vi, n}};
-            g2p_gather(sc, s, acc, nv, nC);
+            if (stencil_in_tile(s, x0, y0, z0)) {
+                TileV g{tv, x0, y0, z0};
+                g2p_gather(sc, s, g, nv, nC);
+            } else {
+                GatherVout g{vi, n};
+                g2p_gather(sc, s, g, nv, nC);
+            }
             if (ch.species == LIQUID) {
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
@@ -442,7 +459,8 @@ __device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __re
 
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ st, const float* __restrict__ adj,
-                                                      const Chunk* __restrict__ chunks, int nchunks,
+                                                      const Chunk* __restrict__ chunks, const int* __restrict__ rows,
+                                                      int nchunks,
                                                       const float4* __restrict__ grid,
                                                       const double* __restrict__ dvbar, int T, int f,
                                                       float4* __restrict__ staging, float4* __restrict__ fallback,
@@ -450,12 +468,12 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
 {
     __shared__ float tv[3 * TILE];
     __shared__ float wt[WAVES * 3 * TILE];
-    __shared__ int own[WAVES * TNODES];
+    __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int* ow = own + wave * TNODES;
+    unsigned char* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
@@ -473,13 +491,12 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
         WaveScatter3 acc{wt + wave * 3 * TILE, x0, y0, z0};
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
-            const int idx = r * WG + threadIdx.x;
-            const bool valid = idx < ch.count;
+            int p = 0;
+            const bool valid = chunk_slot(ch, rows, r * WAVES + wave, lane, p);
             Stencil s;
             float Cb[9], vt[3] = {vstick[0], vstick[1], vstick[2]};
             bool it = false;
             if (valid) {
-                const int p = ch.start + idx;
                 float x[3];
 #pragma unroll
                 for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
@@ -494,8 +511,13 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
                         xa[a] = adj[(SX + a) * LP + p];
                         vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
                     }
-                    TileGatherVfromP gv{tv, x0, y0, z0, it, GatherVfromP{gi, sc}};
-                    g2p_adj_xgrad(sc, s, vt, Cb, gv, xb);
+                    if (it) {
+                        TileV gv{tv, x0, y0, z0};
+                        g2p_adj_xgrad(sc, s, vt, Cb, gv, xb);
+                    } else {
+                        GatherVfromP gv{gi, sc};
+                        g2p_adj_xgrad<true>(sc, s, vt, Cb, gv, xb);
+                    }
 #pragma unroll
                     for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
                 }
@@ -508,10 +530,10 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
             const int key = pending ? node_key(s, x0, y0, z0) : 0;
             while (__ballot(pending)) {
                 lds_order();
-                if (pending) ow[key] = lane;
+                if (pending) ow[key] = (unsigned char)lane;
                 lds_order();
                 if (pending && ow[key] == lane) {
-                    g2p_adj_scatter(sc, s, vt, Cb, acc);
+                    g2p_adj_scatter(sc, opaque(s), vt, Cb, acc);
                     pending = false;
                 }
             }
@@ -554,7 +576,7 @@ __global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ s
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, const float* __restrict__ ain,
                                                       float* __restrict__ adj, const float* __restrict__ xbar,
-                                                      const Chunk* __restrict__ chunks,
+                                                      const GChunk* __restrict__ chunks,
                                                       int nchunks, const double* __restrict__ params,
                                                       const float4* __restrict__ pbar, double* __restrict__ thbar,
                                                       DevScene sc, int P)
@@ -565,7 +587,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
     const long nn = (long)n * n * n;
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const Chunk ch = chunks[c];
+        const GChunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const float4* qi = pbar + (long)ch.inst * nn;
         for (int e = threadIdx.x; e < TNODES; e += WG) {
@@ -593,7 +615,6 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 Stencil s;
                 stencil(sc, q.x[0], q.x[1], q.x[2], s);
                 float Fnb[9], xp[3], xo[3], vo[3], Co[9], Fo[9];
-                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbar{qi, n}};
                 // gather with the affine stored by the recompute pass, then the FP64 SVD and the
                 // stress adjoint.  The empty asm ties C to the gather's results so the compiler
                 // cannot hoist the (independent) SVD into the gather and keep both live.
@@ -601,7 +622,13 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
 #pragma unroll
                 for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
                 const float u[3] = {q.v[0], q.v[1] + sc.gdt, q.v[2]};
-                p2g_adj_gather(sc, s, A, u, acc, ub, Ab, fxb);
+                if (stencil_in_tile(s, x0, y0, z0)) {
+                    TileP4 g{tq, x0, y0, z0};
+                    p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
+                } else {
+                    GatherPbar g{qi, n};
+                    p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
+                }
 #pragma unroll
                 for (int a = 0; a < 9; a++) asm volatile("" : "+v"(q.C[a]) : "v"(Ab[a]), "v"(ub[a % 3]), "v"(fxb[a % 3]));
 #pragma unroll
@@ -609,7 +636,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
 #pragma unroll
                 for (int a = 0; a < 3; a++) xp[a] = xbar[a * LP + p];
                 LiquidP2G L;
-                liquid_p2g(sc, ip.th, ip.kb, q.C, q.F, L);
+                liquid_p2g<true>(sc, ip.th, ip.kb, q.C, q.F, L);
                 p2g_adj_stress(sc, ip, q, L, ub, Ab, fxb, Fnb, xp, xo, vo, Co, Fo, tb);
 #pragma unroll
                 for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
@@ -626,8 +653,13 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 Stencil s;
                 stencil(sc, x[0], x[1], x[2], s);
                 float Co[9];
-                TileGatherPbar acc{tq, x0, y0, z0, stencil_in_tile(s, x0, y0, z0), GatherPbar{qi, n}};
-                p2g_adj_rigid(sc, species_mass(sc, ch.species), s, acc, Co);
+                if (stencil_in_tile(s, x0, y0, z0)) {
+                    TileP4 g{tq, x0, y0, z0};
+                    p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
+                } else {
+                    GatherPbar g{qi, n};
+                    p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
+                }
 #pragma unroll
                 for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
             }
@@ -690,10 +722,13 @@ __device__ __forceinline__ int wave_incl_scan(int v)
     return v;
 }
 
-// One wavefront per bucket: particle count, chunk count and each cell's offset in the bucket.
+// One wavefront per bucket: particle count, row count (the largest cell count), chunk count
+// (rows_per_chunk rows each), gather-chunk count (gchunk particles each) and each cell's
+// offset in the bucket.
 __global__ void __launch_bounds__(256) k_bucket_totals(const int* __restrict__ cnt, int* __restrict__ cell_off,
-                                                       int* __restrict__ bcount, int* __restrict__ bnch, int NBK,
-                                                       int chunk)
+                                                       int* __restrict__ bcount, int* __restrict__ brows,
+                                                       int* __restrict__ bnch, int* __restrict__ bgch, int NBK,
+                                                       int rows_per_chunk, int gchunk)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int bucket = (int)(t >> 6);
@@ -703,63 +738,75 @@ __global__ void __launch_bounds__(256) k_bucket_totals(const int* __restrict__ c
     const int incl = wave_incl_scan(nc);
     cell_off[(long)bucket * 64 + lane] = incl - nc;
     const int total = __shfl(incl, 63, 64);
+    int mx = nc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) {
         bcount[bucket] = total;
-        bnch[bucket] = (total + chunk - 1) / chunk;
+        brows[bucket] = mx;
+        bnch[bucket] = (mx + rows_per_chunk - 1) / rows_per_chunk;
+        bgch[bucket] = (total + gchunk - 1) / gchunk;
     }
 }
 
-// Exclusive scans of two int arrays by one workgroup of 1024 threads; tot[0], tot[1] get the sums.
-__global__ void __launch_bounds__(1024) k_scan2(const int* __restrict__ a, const int* __restrict__ b,
-                                                int* __restrict__ oa, int* __restrict__ ob, int n,
-                                                int* __restrict__ tot)
+// Exclusive scans of four int arrays a[k] by one workgroup of 1024 threads into o[k]; tot[k]
+// gets the sums.
+struct Scan4 {
+    const int* a[4];
+    int* o[4];
+};
+
+__global__ void __launch_bounds__(1024) k_scan4(Scan4 q, int n, int* __restrict__ tot)
 {
-    __shared__ int wa[16], wb[16];
-    __shared__ int carry[2];
+    __shared__ int ws[4][16];
+    __shared__ int carry[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    if (threadIdx.x < 4) carry[threadIdx.x] = 0;
     __syncthreads();
     for (int base = 0; base < n; base += 1024) {
         const int i = base + threadIdx.x;
-        const int va = i < n ? a[i] : 0, vb = i < n ? b[i] : 0;
-        const int sa = wave_incl_scan(va), sb = wave_incl_scan(vb);
+        int v[4], sc[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = i < n ? q.a[k][i] : 0;
+            sc[k] = wave_incl_scan(v[k]);
+        }
         if (lane == 63) {
-            wa[w] = sa;
-            wb[w] = sb;
+#pragma unroll
+            for (int k = 0; k < 4; k++) ws[k][w] = sc[k];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int ra = carry[0], rb = carry[1];
-            for (int k = 0; k < 16; k++) {
-                const int ta = wa[k], tb2 = wb[k];
-                wa[k] = ra;
-                wb[k] = rb;
-                ra += ta;
-                rb += tb2;
+        if (threadIdx.x < 4) {
+            const int k = threadIdx.x;
+            int run = carry[k];
+            for (int j = 0; j < 16; j++) {
+                const int tj = ws[k][j];
+                ws[k][j] = run;
+                run += tj;
             }
-            carry[0] = ra;
-            carry[1] = rb;
+            carry[k] = run;
         }
         __syncthreads();
         if (i < n) {
-            oa[i] = wa[w] + sa - va;
-            ob[i] = wb[w] + sb - vb;
+#pragma unroll
+            for (int k = 0; k < 4; k++) q.o[k][i] = ws[k][w] + sc[k] - v[k];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        tot[0] = carry[0];
-        tot[1] = carry[1];
-    }
+    if (threadIdx.x < 4) tot[threadIdx.x] = carry[threadIdx.x];
 }
 
-// One wavefront per bucket: its chunk entries, and the round-robin slot of every
-// (cell, rank) pair: slot = sum over earlier rounds of their occupied cells + the occupied
-// cells before this one in this round.
+// One wavefront per bucket: the round-robin slot of every (cell, rank) pair (slot = the
+// particles of earlier rows + the occupied cells before this one in this row), the bucket's
+// row table, its chunk entries (rows_per_chunk rows each) and its gather chunks (gchunk
+// particles each).  rows[k] is a global slot; the entry after a bucket's last row is the next
+// bucket's first slot, written by both.
 __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ cnt, const int* __restrict__ cell_off,
-                                                       const int* __restrict__ bstart, const int* __restrict__ cstart,
-                                                       int* __restrict__ rrpos, Chunk* __restrict__ chunks, int NB,
-                                                       int NBK, int chunk)
+                                                       const int* __restrict__ bstart, const int* __restrict__ rstart,
+                                                       const int* __restrict__ cstart, const int* __restrict__ gstart,
+                                                       int* __restrict__ rrpos, int* __restrict__ rows,
+                                                       Chunk* __restrict__ chunks, GChunk* __restrict__ gchunks,
+                                                       int NB, int NBK, int rows_per_chunk, int gchunk)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int bucket = (int)(t >> 6);
@@ -767,28 +814,28 @@ __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ c
     const int lane = threadIdx.x & 63;
     const int nc = cnt[(long)bucket * 64 + lane];
     if (__ballot(nc > 0) == 0ull) return;
-    int total = nc, mx = nc;
+    int mx = nc;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        total += __shfl_xor(total, o, 64);
-        mx = max(mx, __shfl_xor(mx, o, 64));
-    }
-    const int bs = bstart[bucket];
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    const int bs = bstart[bucket], rs = rstart[bucket], cs = cstart[bucket];
     const int nb3 = NB * NB * NB;
     const int inst = bucket / (3 * nb3), sp = (bucket / nb3) % 3, blk = bucket % nb3;
     const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
-    const int nch = (total + chunk - 1) / chunk;
-    for (int i = lane; i < nch; i += 64) {
-        Chunk c;
-        c.inst = inst;
-        c.species = sp;
-        c.ox = bx * TB;
-        c.oy = by * TB;
-        c.oz = bz * TB;
-        c.start = bs + i * chunk;
-        c.count = min(chunk, total - i * chunk);
-        c.pad = 0;
-        chunks[cstart[bucket] + i] = c;
+    int total = nc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
+    const int ngc = (total + gchunk - 1) / gchunk, gs = gstart[bucket];
+    for (int i = lane; i < ngc; i += 64) {
+        GChunk g;
+        g.inst = inst;
+        g.species = sp;
+        g.ox = bx * TB;
+        g.oy = by * TB;
+        g.oz = bz * TB;
+        g.start = bs + i * gchunk;
+        g.count = min(gchunk, total - i * gchunk);
+        g.pad = 0;
+        gchunks[gs + i] = g;
     }
     const int off = cell_off[(long)bucket * 64 + lane];
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -797,8 +844,24 @@ __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ c
         const bool act = nc > r;
         const unsigned long long m = __ballot(act);
         if (act) rrpos[bs + off + r] = S + __popcll(m & below);
+        if (lane == 0) {
+            rows[rs + r] = bs + S;
+            if (r % rows_per_chunk == 0) {
+                Chunk c;
+                c.inst = inst;
+                c.species = sp;
+                c.ox = bx * TB;
+                c.oy = by * TB;
+                c.oz = bz * TB;
+                c.start = bs + S;
+                c.row0 = rs + r;
+                c.nrows = min(rows_per_chunk, mx - r);
+                chunks[cs + r / rows_per_chunk] = c;
+            }
+        }
         S += __popcll(m);
     }
+    if (lane == 0) rows[rs + mx] = bs + S;
 }
 
 // Move every particle to its sorted slot; src[q] records where slot q came from.

@@ -189,6 +189,16 @@ def svd3(F: np.ndarray):
     return U.reshape(n, 3, 3), S, V.reshape(n, 3, 3)
 
 
+def svd3_values(F: np.ndarray) -> np.ndarray:
+    """The FP64 singular values the forward substep uses (signed, as svd3), [n, 3]."""
+    lib = _lib.load()
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(-1, 9))
+    n = F.shape[0]
+    S = np.zeros((n, 3))
+    _lib.check(lib.ds_svd3_values(_p64(F), n, _p64(S)), None)
+    return S
+
+
 def svd3_backward(U, S, V, gU, gS, gV):
     """FP64 SVD backward on the device (simulation_functions.py:70-87)."""
     lib = _lib.load()

@@ -135,6 +135,10 @@ int ds_kernel_timing(ds_ctx* ctx, double* total_ms, int64_t* launches, int32_t r
  * with F = U diag(S) V^T, det U = det V = +1 and S[2] carrying the sign of det F.
  * (The reference's svd(), simulation_functions.py:55-58, uses ti.svd in f32.) */
 int ds_svd3(const double* F, int32_t n, double* U, double* S, double* V);
+/* The singular values alone, as the forward substep computes them for J = S0 S1 S2
+ * (simulation_functions.py:55-58 and :113): F[n][9] -> S[n][3], same signed convention as
+ * ds_svd3, for tests. */
+int ds_svd3_values(const double* F, int32_t n, double* S);
 /* FP64 SVD backward on the device, for tests: given U, S, V and the adjoints gU, gS (diagonal,
  * [n][3]), gV, writes gF.  simulation_functions.py:61-87 (clamped 1/(s_j^2 - s_i^2)). */
 int ds_svd3_backward(const double* U, const double* S, const double* V, const double* gU,

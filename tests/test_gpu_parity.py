@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from diffstir_amd import scene as S
-from diffstir_amd.engine import Engine, svd3, svd3_backward
+from diffstir_amd.engine import Engine, svd3, svd3_backward, svd3_values
 from diffstir_amd.loss import compute_loss
 from oracle.oracle import Oracle, loss_and_grad
 from tests.helpers import force_err, gold_scene, grad_err
@@ -206,6 +206,19 @@ def test_svd3_matches_lapack():
     ref = np.linalg.svd(F, compute_uv=False)
     np.testing.assert_allclose(np.abs(Sg), ref, atol=1e-13, rtol=1e-12)
     np.testing.assert_allclose(Sg.prod(1), np.linalg.det(F), rtol=1e-11, atol=1e-15)
+
+
+def test_svd3_values_match_lapack_and_the_full_svd():
+    """The values-only SVD of the forward substep: sorted, signed like svd3, S0 S1 S2 = det F."""
+    rng = np.random.default_rng(2)
+    F = _random_F(rng, 512)
+    S = svd3_values(F)
+    _, Sf, _ = svd3(F)
+    np.testing.assert_allclose(S, Sf, atol=1e-13, rtol=1e-12)
+    np.testing.assert_allclose(np.abs(S), np.linalg.svd(F, compute_uv=False), atol=1e-13, rtol=1e-12)
+    np.testing.assert_allclose(S.prod(1), np.linalg.det(F), rtol=1e-11, atol=1e-15)
+    assert np.all(S[:, 0] >= S[:, 1]) and np.all(S[:, 1] >= np.abs(S[:, 2]))
+    assert np.all(np.sign(S[:, 2]) * np.sign(np.linalg.det(F)) >= 0)
 
 
 def test_svd3_backward_matches_torch_autograd():
