@@ -378,6 +378,9 @@ int alloc_mode(ds_ctx* c)
         // chunks of a bucket: ceil(rows / (WAVES * PPT)) <= rows / WAVES + 1, and rows <= particles
         c->maxch = (long)c->P / WAVES + std::min<long>(c->P, c->NBK) + 1;
         c->maxgch = (long)c->P / WG + std::min<long>(c->P, c->NBK) + 1;
+        // staging entries are addressed with 32-bit indices (slot * TNODES + node)
+        if ((c->maxch + c->maxch / 4 + 16) * (long)TNODES > 2147483647L)
+            return fail(c, DS_EINVAL, "too many particles for the tiled kernels in one context");
         MALLOC(c->d_vgrid, G);
         MALLOC(c->d_pbar2, G);
         MALLOC(c->d_pfb, G);

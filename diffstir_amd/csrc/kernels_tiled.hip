@@ -285,33 +285,36 @@ __device__ __forceinline__ void load_neighbour_buckets(int* s_nc, int* s_cs, con
 // Sum, in a fixed order, the staging entries of node (local lx, ly, lz of its 4x4x4 node
 // block) from every chunk whose tile covers it.  Per axis the covering buckets are b-1 and b
 // for local nodes 0..2, b and b+1 for local node 3; the node's index in the tile of bucket
-// b + d is local - 4d + 1.
+// b + d is local - 4d + 1.  The 24 (species, covering bucket) pairs are visited together:
+// round k loads the k-th chunk slot of every pair at once, so a round costs one memory latency.
 __device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __restrict__ staging, const int* s_nc,
                                                    const int* s_cs, int lx, int ly, int lz, int ncomp)
 {
     const int ox = lx < 3 ? -1 : 0, oy = ly < 3 ? -1 : 0, oz = lz < 3 ? -1 : 0;
+    int first[24], nc[24];
+    int kmax = 0;
 #pragma unroll
-    for (int sp = 0; sp < 3; sp++)
+    for (int q = 0; q < 24; q++) {
+        const int sp = q >> 3, dx = ox + ((q >> 2) & 1), dy = oy + ((q >> 1) & 1), dz = oz + (q & 1);
+        const int r = sp * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+        const int e = ((lx - 4 * dx + 1) * TS + (ly - 4 * dy + 1)) * TS + (lz - 4 * dz + 1);
+        nc[q] = s_nc[r];
+        first[q] = s_cs[r] * TNODES + e;
+        kmax = max(kmax, nc[q]);
+    }
+    for (int k = 0; k < kmax; k++) {
+        float4 v[24];
 #pragma unroll
-        for (int ddx = 0; ddx < 2; ddx++)
+        for (int q = 0; q < 24; q++)
+            v[q] = k < nc[q] ? staging[first[q] + k * TNODES] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-            for (int ddy = 0; ddy < 2; ddy++)
-#pragma unroll
-                for (int ddz = 0; ddz < 2; ddz++) {
-                    const int dx = ox + ddx, dy = oy + ddy, dz = oz + ddz;
-                    const int r = sp * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-                    const int nc = s_nc[r];
-                    if (nc == 0) continue;
-                    const int e = ((lx - 4 * dx + 1) * TS + (ly - 4 * dy + 1)) * TS + (lz - 4 * dz + 1);
-                    const float4* sl = staging + (long)s_cs[r] * TNODES + e;
-                    for (int k = 0; k < nc; k++) {
-                        const float4 v = sl[(long)k * TNODES];
-                        acc.x += v.x;
-                        acc.y += v.y;
-                        acc.z += v.z;
-                        if (ncomp == 4) acc.w += v.w;
-                    }
-                }
+        for (int q = 0; q < 24; q++) {
+            acc.x += v[q].x;
+            acc.y += v[q].y;
+            acc.z += v[q].z;
+            if (ncomp == 4) acc.w += v[q].w;
+        }
+    }
 }
 
 // One thread per grid node (a workgroup of 64 = one 4x4x4 node block): the sum of the staging
