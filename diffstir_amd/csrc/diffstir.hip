@@ -98,6 +98,9 @@ struct ds_ctx {
     GChunk* d_gchunks = nullptr;  // ncp x maxgch chunks of the gather-only kernels
     int* d_bgch = nullptr;        // gather chunks per bucket
     int* d_gstart = nullptr;      // first gather chunk of each bucket
+    int* d_cov = nullptr;         // ncp x B x NB^3 node blocks reached by a chunk tile
+    int* d_fbflag = nullptr;      // B x NB^3 node blocks marked by a fall-back scatter this step
+    double* d_j = nullptr;        // P: J of each liquid particle for the next P2G
     int* d_brows = nullptr;       // rows per bucket
     int* d_rstart = nullptr;      // first row of each bucket
     int* d_src = nullptr;         // ncp x P
@@ -316,6 +319,9 @@ void free_mode(ds_ctx* c)
     dfree(c->d_gchunks);
     dfree(c->d_bgch);
     dfree(c->d_gstart);
+    dfree(c->d_cov);
+    dfree(c->d_fbflag);
+    dfree(c->d_j);
     dfree(c->d_brows);
     dfree(c->d_rstart);
     dfree(c->d_src);
@@ -401,6 +407,9 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_bgch, c->NBK);
         MALLOC(c->d_gstart, c->NBK);
         MALLOC(c->d_gchunks, (size_t)c->maxgch * c->ncp);
+        MALLOC(c->d_cov, (size_t)c->B * c->NB * c->NB * c->NB * c->ncp);
+        MALLOC(c->d_fbflag, (size_t)c->B * c->NB * c->NB * c->NB);
+        MALLOC(c->d_j, c->P);
         MALLOC(c->d_brows, c->NBK);
         MALLOC(c->d_rstart, c->NBK);
         MALLOC(c->d_rows, (size_t)(c->P + 1) * c->ncp);
@@ -411,6 +420,7 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_src, (size_t)c->P * c->ncp);
         c->nch_h.assign(c->ncp, 0);
         c->ngch_h.assign(c->ncp, 0);
+        fill_i32(c, c->d_fbflag, (long)c->B * c->NB * c->NB * c->NB, 0);
         fill_zero_f4(c, c->d_pfb, G);
         fill_zero_f4(c, c->d_vbfb, G);
     }
@@ -500,7 +510,7 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     hipLaunchKernelGGL(k_sort_count, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_meta,
                        c->d_groups, c->d_key, c->d_rank, c->d_cnt, c->dsc, c->NB, c->P);
     const int rpc = WAVES * c->ppt;   // rows per chunk
-    const int gch = WG * c->ppt;      // particles per gather chunk
+    const int gch = WG;               // particles per gather chunk
     c->ppt_sorted = c->ppt;
     int* rows = c->d_rows + (size_t)cp * (c->P + 1);
     GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
@@ -508,9 +518,12 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
                        c->d_celloff, c->d_bcount, c->d_brows, bnch, c->d_bgch, c->NBK, rpc, gch);
     Scan4 q{{c->d_bcount, bnch, c->d_brows, c->d_bgch}, {c->d_bstart, cstart, c->d_rstart, c->d_gstart}};
     hipLaunchKernelGGL(k_scan4, dim3(1), dim3(1024), 0, c->stream, q, c->NBK, c->d_tot);
+    const long nbl = (long)c->B * c->NB * c->NB * c->NB;
+    int* cov = c->d_cov + (size_t)cp * nbl;
+    fill_i32(c, cov, nbl, 0);
     hipLaunchKernelGGL(k_bucket_layout, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
                        c->d_celloff, c->d_bstart, c->d_rstart, cstart, c->d_gstart, c->d_rrpos, rows, chunks, gchunks,
-                       c->NB, c->NBK, rpc, gch);
+                       cov, c->NB, c->NBK, rpc, gch);
     fill_i32(c, c->d_meta2, c->P, -1);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
                        c->d_rank, c->d_celloff, c->d_bstart, c->d_rrpos, c->d_meta, c->d_meta2, c->d_pid, c->d_pid2,
@@ -553,13 +566,13 @@ void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* p
         KernelTimer kt(c, DS_K_P2G);
         DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
                         c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1), nch, c->d_params,
-                        c->d_staging, c->d_pfb, c->dsc, c->P,
-                        c->d_flag);
+                        c->d_staging, c->d_pfb, c->d_fbflag, c->NB, c->d_j, c->dsc, c->P, c->d_flag);
     }
     const int nb3 = c->NB * c->NB * c->NB;
     KernelTimer kt(c, DS_K_GRID_REDUCE);
-    hipLaunchKernelGGL(k_grid_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart, c->d_pfb,
-                       pout, c->d_vgrid, c->dsc, c->NB);
+    const int* cov = c->d_cov + (size_t)cp * c->B * nb3;
+    hipLaunchKernelGGL(k_grid_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart, cov,
+                       c->d_pfb, c->d_fbflag, pout, c->d_vgrid, c->dsc, c->NB);
 }
 
 void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool record)
@@ -569,7 +582,16 @@ void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool recor
     KernelTimer kt(c, DS_K_G2P);
     DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_tiled, dim3(ngc), dim3(WG), 0, c->stream, in, out,
                     c->d_gchunks + (size_t)cp * c->maxgch, ngc, c->d_vgrid, c->d_traj, c->T, f,
-                    record ? c->d_dvsum : nullptr, c->dsc, c->P, c->d_flag);
+                    record ? c->d_dvsum : nullptr, c->d_j, c->dsc, c->P, c->d_flag);
+}
+
+// J of every liquid particle of the checkpoint state `st` into d_j, for the segment's first P2G.
+void liquid_j_tiled(ds_ctx* c, const float* st, int cp)
+{
+    const int ngc = c->ngch_h[cp];
+    if (ngc == 0) return;
+    DS_PPT_DISPATCH(c->ppt_sorted, k_liquid_j, dim3(ngc), dim3(WG), 0, c->stream, st,
+                    c->d_gchunks + (size_t)cp * c->maxgch, ngc, c->d_j, c->dsc, c->P);
 }
 
 int forward_tiled(ds_ctx* c, int nsteps)
@@ -586,6 +608,7 @@ int forward_tiled(ds_ctx* c, int nsteps)
                 RCHK(sort_into_checkpoint(c, cp));
             }
             in = c->d_ckpt + (size_t)cp * SF;
+            liquid_j_tiled(c, in, cp);
         }
         p2g_reduce_tiled(c, in, c->d_state, cp, nullptr, nullptr);
         g2p_tiled(c, in, c->d_state, cp, s, true);
@@ -616,6 +639,7 @@ int backward_tiled(ds_ctx* c)
         auto grid_of = [&](int k) -> float4* { return c->d_seg_grid + (size_t)k * G; };
         auto aff_of = [&](int k) -> float* { return c->d_seg_aff + (size_t)k * 9 * c->P; };
         // recompute the segment, keeping each step's state and momentum/mass grid
+        liquid_j_tiled(c, ck, cp);
         for (int k = 0; k < L; k++) {
             const int s = s0 + k;
             c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
@@ -629,13 +653,14 @@ int backward_tiled(ds_ctx* c)
             if (nch > 0) {
                 KernelTimer kt(c, DS_K_G2P_ADJ);
                 DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
-                                chunks, rows, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb, c->d_xbar,
-                                c->dsc, c->P);
+                                chunks, rows, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb,
+                                c->d_fbflag, c->NB, c->d_xbar, c->dsc, c->P);
             }
             {
                 KernelTimer kt(c, DS_K_VBAR_REDUCE);
                 hipLaunchKernelGGL(k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
-                                   c->d_vbfb, grid_of(k), c->d_pbar2, c->dsc, c->NB);
+                                   c->d_cov + (size_t)cp * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2,
+                                   c->dsc, c->NB);
             }
             KernelTimer kt(c, DS_K_P2G_ADJ);
             if (ngc > 0)

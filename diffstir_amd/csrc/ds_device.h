@@ -393,9 +393,8 @@ struct LiquidP2G {
     double U[9], S[3], V[9];
 };
 
-template <bool kFull>
-__device__ __forceinline__ void liquid_p2g(const DevScene& sc, const float th[4], double kb, const float C[9],
-                                           const float F[9], LiquidP2G& q)
+// F_tmp = (I + dt C) F in FP32 (simulation_functions.py:39-42), the F of the next step.
+__device__ __forceinline__ void f_tmp(const DevScene& sc, const float C[9], const float F[9], float Fn[9])
 {
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -404,16 +403,24 @@ __device__ __forceinline__ void liquid_p2g(const DevScene& sc, const float th[4]
             float s = F[i * 3 + j];
 #pragma unroll
             for (int k = 0; k < 3; k++) s = fmaf(sc.dt * C[i * 3 + k], F[k * 3 + j], s);
-            q.Fn[i * 3 + j] = s;
+            Fn[i * 3 + j] = s;
         }
-    double Fd[9];
+}
+
+// J = S0 S1 S2 of the FP64 SVD of F_tmp (the values-only form).
+__device__ __forceinline__ double liquid_j(const float Fn[9])
+{
+    double Fd[9], S[3];
 #pragma unroll
-    for (int i = 0; i < 9; i++) Fd[i] = (double)q.Fn[i];
-    if (kFull)
-        svd3(Fd, q.U, q.S, q.V);
-    else
-        svd3_values(Fd, q.S);
-    double J = q.S[0] * q.S[1] * q.S[2];
+    for (int i = 0; i < 9; i++) Fd[i] = (double)Fn[i];
+    svd3_values(Fd, S);
+    return S[0] * S[1] * S[2];
+}
+
+// The stress half of liquid_p2g for a given J: gamma, eta(gamma) and A.
+__device__ __forceinline__ void liquid_stress(const DevScene& sc, const float th[4], double kb, const float C[9],
+                                              double J, LiquidP2G& q)
+{
     q.J = J;
     float n2 = 0.0f;
 #pragma unroll
@@ -429,6 +436,24 @@ __device__ __forceinline__ void liquid_p2g(const DevScene& sc, const float th[4]
             float s = q.eta * 0.5f * (C[i * 3 + j] + C[j * 3 + i]) + (i == j ? pr : 0.0f);
             q.A[i * 3 + j] = sc.alpha * s + sc.m_liq * C[i * 3 + j];
         }
+}
+
+template <bool kFull>
+__device__ __forceinline__ void liquid_p2g(const DevScene& sc, const float th[4], double kb, const float C[9],
+                                           const float F[9], LiquidP2G& q)
+{
+    f_tmp(sc, C, F, q.Fn);
+    double J;
+    if (kFull) {
+        double Fd[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) Fd[i] = (double)q.Fn[i];
+        svd3(Fd, q.U, q.S, q.V);
+        J = q.S[0] * q.S[1] * q.S[2];
+    } else {
+        J = liquid_j(q.Fn);
+    }
+    liquid_stress(sc, th, kb, C, J, q);
 }
 
 // ---------------------------------------------------------------------------- grid update

@@ -49,14 +49,21 @@ struct P2GPayload {
     float m;
 };
 
+// Jin: J of the particle when already known (computed from the same F_tmp by the previous
+// step's G2P), else nullptr to compute it here.
 __device__ __forceinline__ void p2g_prepare(const DevScene& sc, int sp, const InstParams& ip, const ParticleIn& q,
-                                            float Fn[9], P2GPayload& pl)
+                                            float Fn[9], P2GPayload& pl, const double* Jin = nullptr)
 {
     float u[3];
     pl.m = species_mass(sc, sp);
     if (sp == LIQUID) {
         LiquidP2G L;
-        liquid_p2g<false>(sc, ip.th, ip.kb, q.C, q.F, L);
+        if (Jin) {
+            f_tmp(sc, q.C, q.F, L.Fn);
+            liquid_stress(sc, ip.th, ip.kb, q.C, *Jin, L);
+        } else {
+            liquid_p2g<false>(sc, ip.th, ip.kb, q.C, q.F, L);
+        }
 #pragma unroll
         for (int a = 0; a < 9; a++) {
             Fn[a] = L.Fn[a];

@@ -99,6 +99,19 @@ __device__ __forceinline__ Stencil opaque(const Stencil& s)
     return t;
 }
 
+// Marks the node blocks a stencil touches (its corner nodes' blocks) in the per-step fall-back
+// flags, so the reduction visits them even where no chunk tile covers them.
+__device__ __forceinline__ void mark_fallback_blocks(int* __restrict__ fbflag, int inst, const Stencil& s, int NB)
+{
+    const int nb3 = NB * NB * NB;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int bx = (s.b[0] + ((q >> 2) & 1) * 2) / TB, by = (s.b[1] + ((q >> 1) & 1) * 2) / TB,
+                  bz = (s.b[2] + (q & 1) * 2) / TB;
+        atomicOr(fbflag + (long)inst * nb3 + (bx * NB + by) * NB + bz, 1);
+    }
+}
+
 // Read-modify-write scatter into a wavefront's private tile (see the claim protocol above).
 // The barrier after each node keeps the next node's reads behind this node's writes: another
 // lane's stencil may reach the same node at a different offset.
@@ -166,9 +179,11 @@ __device__ __forceinline__ void load_particle_t(const float* __restrict__ st, lo
     for (int a = 0; a < 9; a++) q.F[a] = with_F ? st[(SF + a) * LP + p] : ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f);
 }
 
+// Zeroes nfloats (a multiple of 4) of a 16-byte aligned LDS array with 16-byte stores.
 __device__ __forceinline__ void zero_lds(float* t, int nfloats)
 {
-    for (int e = threadIdx.x; e < nfloats; e += WG) t[e] = 0.0f;
+    float4* t4 = reinterpret_cast<float4*>(t);
+    for (int e = threadIdx.x; e < nfloats / 4; e += WG) t4[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __device__ __forceinline__ int node_key(const Stencil& s, int x0, int y0, int z0)
@@ -199,36 +214,43 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
                                                   const int* __restrict__ rows,
                                                   int nchunks,
                                                   const double* __restrict__ params, float4* __restrict__ staging,
-                                                  float4* __restrict__ fallback, DevScene sc, int P,
+                                                  float4* __restrict__ fallback, int* __restrict__ fbflag, int NB,
+                                                  const double* __restrict__ jin, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
 {
-    __shared__ float wt[WAVES * 4 * TILE];
+    __shared__ __align__(16) float wt[WAVES * 4 * TILE];
     __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     unsigned char* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        zero_lds(wt, WAVES * 4 * TILE);
-        __syncthreads();
         const Chunk ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
+        const bool liq = ch.species == LIQUID;
+        // the first row's particles are loaded before the tile is cleared, and each next row's
+        // while the current one scatters
+        int p = 0;
+        bool valid = chunk_slot(ch, rows, wave, lane, p);
+        ParticleIn q;
+        double J = 1.0;
+        if (valid) {
+            load_particle_t(in, LP, p, liq, q);
+            if (liq) J = jin[p];
+        }
+        zero_lds(wt, WAVES * 4 * TILE);
         const InstParams ip = load_params(sc, params, ch.inst);
         WaveScatter4 acc{wt + wave * 4 * TILE, x0, y0, z0};
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
-            int p = 0;
-            const bool valid = chunk_slot(ch, rows, r * WAVES + wave, lane, p);
             Stencil s;
             P2GPayload pl;
             bool it = false;
             if (valid) {
-                ParticleIn q;
-                load_particle_t(in, LP, p, ch.species == LIQUID, q);
                 if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
                 float Fn[9];
-                p2g_prepare(sc, ch.species, ip, q, Fn, pl);
+                p2g_prepare(sc, ch.species, ip, q, Fn, pl, &J);
                 if (fout && ch.species == LIQUID) {
 #pragma unroll
                     for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
@@ -239,10 +261,18 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
                 }
                 it = stencil_in_tile(s, x0, y0, z0);
                 if (!it) {
+                    mark_fallback_blocks(fbflag, ch.inst, s, NB);
                     ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
                     p2g_scatter(sc, s, pl, g);
                 }
             }
+            int pn = 0;
+            const bool vn = r + 1 < PPT && chunk_slot(ch, rows, (r + 1) * WAVES + wave, lane, pn);
+            if (vn) {
+                load_particle_t(in, LP, pn, liq, q);
+                if (liq) J = jin[pn];
+            }
+            if (r == 0) __syncthreads();   // every wavefront's tile is cleared
             bool pending = valid && it;
             const int key = pending ? node_key(s, x0, y0, z0) : 0;
             while (__ballot(pending)) {
@@ -254,6 +284,8 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
                     pending = false;
                 }
             }
+            valid = vn;
+            p = pn;
         }
         __syncthreads();
         store_staging(wt, 4 * TILE, 4, staging + (long)c * TNODES);
@@ -321,11 +353,18 @@ __device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __
 // slots of the chunks whose tiles cover the node, in a fixed order, plus the fall-back grid
 // (read and cleared for the next step); then the grid update.  Writes the velocity (vout) and,
 // when pout is given, the momentum/mass.
+// Only blocks that a chunk tile reaches (cov, fixed for a checkpoint segment) or that a
+// fall-back scatter marked this step (fbflag, cleared here) are visited.  The others keep stale
+// values that no kernel reads: gathers read nodes inside a chunk's tile or inside the stencil of
+// a particle that scattered into them in the same step.
 __global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ staging, const int* __restrict__ bnch,
-                                                    const int* __restrict__ cstart, float4* __restrict__ fallback,
+                                                    const int* __restrict__ cstart, const int* __restrict__ cov,
+                                                    float4* __restrict__ fallback, int* __restrict__ fbflag,
                                                     float4* __restrict__ pout, float4* __restrict__ vout, DevScene sc,
                                                     int NB)
 {
+    const int fl = fbflag[blockIdx.x];
+    if (!cov[blockIdx.x] && !fl) return;
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const int nb3 = NB * NB * NB;
@@ -338,8 +377,12 @@ __global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ s
     __syncthreads();
     if (gx >= n || gy >= n || gz >= n) return;
     const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
-    float4 acc = fallback[gid];
-    fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (fl) {
+        acc = fallback[gid];
+        fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (threadIdx.x == 0) fbflag[blockIdx.x] = 0;
+    }
     sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, 4);
     if (pout) pout[gid] = acc;
     float v[3];
@@ -352,8 +395,8 @@ template <int PPT>
 __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
                                                   const GChunk* __restrict__ chunks, int nchunks,
                                                   const float4* __restrict__ vgrid, const float* __restrict__ traj,
-                                                  int T, int f, double* __restrict__ dvsum, DevScene sc, int P,
-                                                  unsigned* __restrict__ flag)
+                                                  int T, int f, double* __restrict__ dvsum, double* __restrict__ jout,
+                                                  DevScene sc, int P, unsigned* __restrict__ flag)
 {
     __shared__ float tv[3 * TILE];
     __shared__ double red[WAVES][3];
@@ -382,9 +425,15 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
             const int idx = r * WG + threadIdx.x;
             if (idx >= ch.count) break;
             const int p = ch.start + idx;
-            float x[3];
+            float x[3], Fo[9];
 #pragma unroll
             for (int a = 0; a < 3; a++) x[a] = in[(SX + a) * LP + p];
+            // F of the next step, written to out by this step's P2G (read before out is written)
+            const bool want_j = jout && ch.species == LIQUID;
+            if (want_j) {
+#pragma unroll
+                for (int a = 0; a < 9; a++) Fo[a] = out[(SF + a) * LP + p];
+            }
             Stencil s;
             if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
             float nv[3], nC[9];
@@ -400,6 +449,12 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
                 for (int a = 0; a < 3; a++) {
                     out[(SV + a) * LP + p] = nv[a];
                     out[(SX + a) * LP + p] = x[a] + sc.dt * nv[a];
+                }
+                if (want_j) {
+                    // J of the next step's P2G: F_tmp = (I + dt C') F'
+                    float Fn[9];
+                    f_tmp(sc, nC, Fo, Fn);
+                    jout[p] = liquid_j(Fn);
                 }
             } else if (ch.species == STICK) {
 #pragma unroll
@@ -438,6 +493,33 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
     }
 }
 
+// J of every liquid particle of a state (F_tmp = (I + dt C) F, values-only FP64 SVD), for the
+// first P2G after a checkpoint; later steps get J from the previous step's G2P.
+template <int PPT>
+__global__ void __launch_bounds__(WG) k_liquid_j(const float* __restrict__ st, const GChunk* __restrict__ chunks,
+                                                 int nchunks, double* __restrict__ jout, DevScene sc, int P)
+{
+    const long LP = P;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const GChunk ch = chunks[c];
+        if (ch.species != LIQUID) continue;
+#pragma unroll 1
+        for (int r = 0; r < PPT; r++) {
+            const int idx = r * WG + threadIdx.x;
+            if (idx >= ch.count) break;
+            const int p = ch.start + idx;
+            float C[9], F[9], Fn[9];
+#pragma unroll
+            for (int a = 0; a < 9; a++) {
+                C[a] = st[(SC + a) * LP + p];
+                F[a] = st[(SF + a) * LP + p];
+            }
+            f_tmp(sc, C, F, Fn);
+            jout[p] = liquid_j(Fn);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------- adjoint
 
 // Velocity tile from the momentum/mass grid: the grid update at every node of the tile.
@@ -467,15 +549,16 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
                                                       const float4* __restrict__ grid,
                                                       const double* __restrict__ dvbar, int T, int f,
                                                       float4* __restrict__ staging, float4* __restrict__ fallback,
-                                                      float* __restrict__ xbar, DevScene sc, int P)
+                                                      int* __restrict__ fbflag, int NB, float* __restrict__ xbar,
+                                                      DevScene sc, int P)
 {
     __shared__ float tv[3 * TILE];
-    __shared__ float wt[WAVES * 3 * TILE];
+    __shared__ __align__(16) float wt[WAVES * 3 * TILE];
     __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     unsigned char* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const Chunk ch = chunks[c];
@@ -525,6 +608,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
                     for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
                 }
                 if (!it) {
+                    mark_fallback_blocks(fbflag, ch.inst, s, NB);
                     ScatterGlobal3 g{fallback + (long)ch.inst * nn, n};
                     g2p_adj_scatter(sc, s, vt, Cb, g);
                 }
@@ -551,10 +635,13 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
 // fall-back grid, cleared for the next step), then the adjoint of the grid update with the
 // node's momentum/mass -> dL/d(momentum, mass) into pbar.
 __global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ staging, const int* __restrict__ bnch,
-                                                    const int* __restrict__ cstart, float4* __restrict__ fallback,
+                                                    const int* __restrict__ cstart, const int* __restrict__ cov,
+                                                    float4* __restrict__ fallback, int* __restrict__ fbflag,
                                                     const float4* __restrict__ grid, float4* __restrict__ pbar,
                                                     DevScene sc, int NB)
 {
+    const int fl = fbflag[blockIdx.x];
+    if (!cov[blockIdx.x] && !fl) return;
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const int nb3 = NB * NB * NB;
@@ -567,8 +654,12 @@ __global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ s
     __syncthreads();
     if (gx >= n || gy >= n || gz >= n) return;
     const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
-    float4 acc = fallback[gid];
-    fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (fl) {
+        acc = fallback[gid];
+        fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (threadIdx.x == 0) fbflag[blockIdx.x] = 0;
+    }
     sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, 3);
     float v[3];
     NodeRec r;
@@ -809,7 +900,8 @@ __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ c
                                                        const int* __restrict__ cstart, const int* __restrict__ gstart,
                                                        int* __restrict__ rrpos, int* __restrict__ rows,
                                                        Chunk* __restrict__ chunks, GChunk* __restrict__ gchunks,
-                                                       int NB, int NBK, int rows_per_chunk, int gchunk)
+                                                       int* __restrict__ cov, int NB, int NBK, int rows_per_chunk,
+                                                       int gchunk)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int bucket = (int)(t >> 6);
@@ -824,6 +916,12 @@ __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ c
     const int nb3 = NB * NB * NB;
     const int inst = bucket / (3 * nb3), sp = (bucket / nb3) % 3, blk = bucket % nb3;
     const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
+    if (lane < 27) {
+        // the node blocks this bucket's tiles reach
+        const int cx = bx + lane / 9 - 1, cy = by + (lane / 3) % 3 - 1, cz = bz + lane % 3 - 1;
+        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < NB && cy < NB && cz < NB)
+            cov[(long)inst * nb3 + (cx * NB + cy) * NB + cz] = 1;
+    }
     int total = nc;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
