@@ -35,14 +35,17 @@ HBM_PEAK_GBS = 8000.0
 FWD_BYTES = 192.0            # FP32 x, v, C, F read once and written once per particle-step
 
 # Algorithmic HBM bytes per particle (all species) of one launch of each tiled kernel: the
-# FP32 particle data the kernel must read and write.  P2G and G2P together cover the 192 B
-# forward particle-step (x is read by both, hence 204).
+# particle data the kernel must read and write (FP32 state, FP64 J and SVD factor M).
 KERNEL_BYTES_PER_PARTICLE = {
-    "k_p2g_tiled": 96.0 + 36.0,          # read x, v, C, F; write F
-    "k_g2p_tiled": 12.0 + 60.0,          # read x; write x, v, C
-    "k_g2p_adj_tiled": 12.0 + 72.0 + 12.0,          # read x and the adjoints of x, v, C; write dL/dx part
-    "k_p2g_adj_tiled": 96.0 + 36.0 + 36.0 + 12.0 + 96.0,  # read x, v, C, F, affine, adjoint F, dL/dx part; write adjoint
+    "k_p2g_tiled": 96.0 + 8.0 + 36.0,                     # read x, v, C, F, J; write F
+    "k_g2p_tiled": 12.0 + 36.0 + 60.0 + 8.0,              # read x, F; write x, v, C, J
+    "k_g2p_tiled_factor": 12.0 + 36.0 + 60.0 + 8.0 + 80.0,  # ... and write J, M (the recompute)
+    "k_g2p_adj_tiled": 12.0 + 60.0,                       # read x and the adjoints of x, v, C
+    "k_p2g_adj_stress": 72.0 + 72.0 + 36.0 + 80.0 + 96.0,  # read C, F, gathered, adj F, J/M; write adj
 }
+# the adjoint P2G gathers: read x, v, A and the adjoints of x, v, C; then either the stress half
+# too (fused: read C, F, adj F, J/M, write the adjoint) or the 18 gathered floats (split)
+ADJ_P2G_BYTES = {"fused": 120.0 + 188.0 + 96.0, "split": 120.0 + 72.0}
 
 
 def fwd_adj_bytes(K: int) -> float:
@@ -79,7 +82,8 @@ def grad_eval(eng: Engine, targets, kind):
     return losses, d
 
 
-def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10):
+def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10,
+            autotune: bool = True):
     scenes = S.workload(workload)
     n_liq = sum(s.n_liquid for s in scenes)
     n_all = sum(s.n_liquid + s.n_stick + s.n_container for s in scenes)
@@ -87,6 +91,12 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
     log(f"[bench] workload {workload}: {len(scenes)} instance(s), {n_liq} liquid particles, {T - 1} substeps, "
         f"checkpoint every {ckpt}")
     eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels, kernel_timing_every=timing_every if kernels == "tiled" else 0)
+    tuned = None
+    if autotune and kernels == "tiled":
+        from diffstir_amd.autotune import tune
+        t0 = time.perf_counter()
+        tuned = tune(scenes, ckpt_every=ckpt, engine=eng)
+        log(f"[bench] autotuned in {time.perf_counter() - t0:.1f} s: {tuned['options']}")
     targets = make_targets(eng, scenes)
     eng.set_params(candidate_params(scenes))
     kind = scenes[0].cfg.loss_kind
@@ -108,8 +118,9 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
     if not np.all(np.isfinite(d)):
         raise RuntimeError("non-finite gradient")
     psteps = float(n_liq) * (T - 1) * steps
-    res = dict(workload=workload, scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps, fwd_ms=fwd_ms / steps,
-               bwd_ms=bwd_ms / steps, B=len(scenes), kernels=eng.kernel_timing())
+    res = dict(workload=workload, scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps,
+               fwd_ms=fwd_ms / steps, bwd_ms=bwd_ms / steps, B=len(scenes), kernels=eng.kernel_timing(),
+               options=eng.options() if kernels == "tiled" else None, tuned=tuned)
     eng.close()
     return res
 
@@ -134,13 +145,16 @@ def cpu_baseline(workload: str, sample_steps: int, ckpt: int):
 def roofline(r) -> dict:
     """The dominant tiled kernel (largest sampled GPU time): its algorithmic bytes per launch
     divided by its average launch time, both from this run."""
-    kt = {k: v for k, v in r["kernels"].items() if k in KERNEL_BYTES_PER_PARTICLE and v[1] > 0}
+    bpp = dict(KERNEL_BYTES_PER_PARTICLE)
+    if r.get("options"):
+        bpp["k_p2g_adj_tiled"] = ADJ_P2G_BYTES[r["options"]["adjoint_p2g"]]
+    kt = {k: v for k, v in r["kernels"].items() if k in bpp and v[1] > 0}
     if not kt:
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None}
     name = max(kt, key=lambda k: kt[k][0])
     tot_ms, n = kt[name]
     avg_s = tot_ms / n / 1e3
-    per_launch = KERNEL_BYTES_PER_PARTICLE[name] * r["n_all"]
+    per_launch = bpp[name] * r["n_all"]
     achieved = per_launch / avg_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{r['workload']}.json")
@@ -180,12 +194,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="single_n64")
     ap.add_argument("--ckpt", type=int, default=50)
-    ap.add_argument("--cpu-sample-steps", type=int, default=12)
+    ap.add_argument("--cpu-sample-steps", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels", default="tiled", choices=["tiled", "per_particle"])
     ap.add_argument("--batch-workload", default="batch32_n64")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
     a = ap.parse_args()
     if a.calibrate_copy_mib:
         calibrate_copy(a.calibrate_copy_mib)
@@ -194,7 +209,7 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the instances of a batch share one device "
               "and nothing here uses a second GPU", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels)
+    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune)
     value = r["psteps"] / r["elapsed"]
     ms_per_step = 1000.0 * r["elapsed"] / a.steps
     bpps = fwd_adj_bytes(a.ckpt)
@@ -214,7 +229,7 @@ def main():
         "data": "synthetic (seeded cylinder fill, circular stick path)",
         "config": {"workload": a.workload, "instances": r["B"], "liquid_particles": r["n_liq"],
                    "substeps": r["T"] - 1, "n_grid": r["scenes"][0].cfg.n_grid, "ckpt_every": a.ckpt,
-                   "kernels": a.kernels},
+                   "kernels": a.kernels, "options": r["options"], "autotuned": r["tuned"] is not None},
         "extra": {"gradient_evals_per_s": r["B"] * a.steps / r["elapsed"], "fwd_ms": r["fwd_ms"],
                   "adjoint_ms": r["bwd_ms"],
                   "whole_evaluation_GBps": achieved,
@@ -222,9 +237,10 @@ def main():
         "roofline": roofline(r),
     }
     if not a.no_batch:
-        rb = run_gpu(a.batch_workload, 1, 1, a.ckpt, a.kernels)
+        rb = run_gpu(a.batch_workload, 1, 1, a.ckpt, a.kernels, autotune=not a.no_autotune)
         line["extra"]["batched"] = {
             "workload": a.batch_workload, "instances": rb["B"], "liquid_particles": rb["n_liq"],
+            "options": rb["options"],
             "gradient_evals_per_s": rb["B"] / rb["elapsed"],
             "fwd_adjoint_particle_steps_per_s": rb["psteps"] / rb["elapsed"],
             "ratio_to_single_instance_evals": (rb["B"] / rb["elapsed"]) / line["extra"]["gradient_evals_per_s"],

@@ -1,0 +1,174 @@
+"""Autotuner for the execution options of the tiled kernels.
+
+The options change how a substep runs on the device, never what it computes:
+
+* ``particles_per_thread`` -- rows per wavefront of the scatter kernels (P2G and the adjoint G2P);
+  a chunk is 4 * ppt rows of one bucket, a row holding at most one particle per cell;
+* ``gather_particles_per_thread`` -- particles per thread of the gather kernels (G2P, J, the
+  adjoint P2G); a gather chunk is 256 * gppt particles of one bucket;
+* ``adjoint_p2g`` -- the adjoint P2G as one kernel ("fused") or as an LDS-gather kernel followed
+  by a per-particle stress kernel ("split");
+* ``scatter_chunks`` -- the scatter kernels on rows ("rows": lanes of a wavefront never share a
+  stencil base, but rows of sparse buckets leave lanes idle) or on the packed gather chunks
+  ("packed": full wavefronts, lanes sharing a base served in successive passes).  With packed
+  chunks the rows-per-wavefront choice does not apply.
+
+Which choice is fastest depends on the workload's size and on how the particles cluster, so each
+candidate is timed on the workload itself: a short checkpointed forward + adjoint rollout
+(``substeps`` substeps of every instance) on one Engine, best of ``repeats``.  The particles of a
+freshly filled container sit on a regular lattice, which is not how they are spread over most of
+a rollout, so the timed rollout starts from the positions the workload reaches after
+``warm_steps`` substeps (half of the rollout by default; velocities restart from rest).  Results
+are cached by a workload signature, in memory and optionally in a JSON file.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from typing import Callable, Dict, Iterable, List, Optional, Sequence
+
+import numpy as np
+
+SPACE = {
+    "particles_per_thread": (1, 2, 4),
+    "gather_particles_per_thread": (1, 2),
+    "adjoint_p2g": ("fused", "split"),
+    "scatter_chunks": ("rows", "packed"),
+}
+DEFAULT = {"particles_per_thread": 2, "gather_particles_per_thread": 1, "adjoint_p2g": "fused",
+           "scatter_chunks": "rows"}
+
+_CACHE: Dict[str, dict] = {}
+
+
+def candidates(space: Optional[dict] = None) -> List[dict]:
+    """Every distinct combination of the option values in `space` (default SPACE): with packed
+    scatter chunks, particles_per_thread (rows per wavefront) has no effect and keeps its default."""
+    space = space or SPACE
+    keys = list(space)
+    out, seen = [], set()
+    for vals in itertools.product(*(space[k] for k in keys)):
+        d = dict(zip(keys, vals))
+        if d.get("scatter_chunks") == "packed" and "particles_per_thread" in d:
+            d["particles_per_thread"] = DEFAULT["particles_per_thread"]
+        key = tuple(sorted(d.items()))
+        if key not in seen:
+            seen.add(key)
+            out.append(d)
+    return out
+
+
+def signature(scenes, ckpt_every: int) -> str:
+    """What the choice depends on: grid size, instance count, particle counts (to 2 digits), K."""
+    n = scenes[0].cfg.n_grid
+    total = sum(s.n_liquid + s.n_stick + s.n_container for s in scenes)
+    return f"n{n}_b{len(scenes)}_p{float(f'{total:.2g}'):g}_k{ckpt_every}"
+
+
+def pick(timings: Iterable[tuple]) -> dict:
+    """The candidate with the smallest time; ties go to the earlier candidate."""
+    best, best_t = None, float("inf")
+    for cand, t in timings:
+        if np.isfinite(t) and t < best_t:
+            best, best_t = cand, t
+    if best is None:
+        raise RuntimeError("no candidate ran")
+    return dict(best)
+
+
+def time_candidate(engine, cand: dict, substeps: int, repeats: int = 2) -> float:
+    """Milliseconds of a forward + adjoint rollout of `substeps` substeps with options `cand`
+    (the library's own HIP-event timing), best of `repeats`."""
+    engine.configure(**cand)
+    best = float("inf")
+    for _ in range(repeats):
+        engine.reset()
+        engine.forward(substeps)
+        F = engine.forces()
+        engine.backward(np.ones_like(F))
+        fwd_ms, adj_ms = engine.timing()
+        best = min(best, fwd_ms + adj_ms)
+    return best
+
+
+def warmed_scenes(scenes, engine, warm_steps: int):
+    """`scenes` restarted from the particle positions `engine` (an Engine over `scenes`, default
+    options) reaches after `warm_steps` substeps: liquid and stick positions of that step, the
+    stick trajectory from that frame on."""
+    import dataclasses
+    engine.configure(**DEFAULT)
+    engine.reset()
+    engine.forward(warm_steps)
+    out = []
+    for b, s in enumerate(scenes):
+        xl = engine.read_liquid(b)["x"]
+        xs = engine.read_stick(b)["x"] if s.n_stick else s.x_stick
+        out.append(dataclasses.replace(s, x_liquid=xl, x_stick=xs,
+                                       traj=np.ascontiguousarray(s.traj[warm_steps:])))
+    return out
+
+
+def tune(scenes, ckpt_every: int = 50, substeps: Optional[int] = None, space: Optional[dict] = None,
+         repeats: int = 2, cache_file: Optional[str] = None, engine=None, warm_steps: Optional[int] = None,
+         timer: Optional[Callable[[object, dict, int, int], float]] = None, verbose: bool = False) -> dict:
+    """Choose the execution options for `scenes`; returns {"options": ..., "timings_ms": [...],
+    "signature": ..., "substeps": ..., "warm_steps": ...}.  `engine` (an Engine over `scenes`) is
+    used when given (its scenes are restored afterwards and the chosen options applied), else a
+    temporary one."""
+    scenes = list(scenes)
+    sig = signature(scenes, ckpt_every)
+    if sig in _CACHE:
+        return _CACHE[sig]
+    if cache_file and os.path.exists(cache_file):
+        with open(cache_file) as f:
+            disk = json.load(f)
+        if sig in disk:
+            _CACHE[sig] = disk[sig]
+            return disk[sig]
+    from .engine import Engine
+    T = min(s.max_steps for s in scenes)
+    if substeps is None:
+        substeps = min(T - 1, 2 * ckpt_every)
+    if warm_steps is None:
+        warm_steps = (T - 1) // 2
+    warm_steps = max(0, min(warm_steps, T - 1 - substeps))
+    timer = timer or time_candidate
+    own = engine is None
+    eng = Engine(scenes, ckpt_every=ckpt_every) if own else engine
+    try:
+        if warm_steps > 0:
+            # time on the warmed positions in the same context, then put the scenes back
+            for b, s in enumerate(warmed_scenes(scenes, eng, warm_steps)):
+                eng.upload_scene(b, s)
+        timings = []
+        for cand in candidates(space):
+            t = timer(eng, cand, substeps, repeats)
+            timings.append((cand, t))
+            if verbose:
+                print(f"[autotune] {cand}: {t:.2f} ms", flush=True)
+        best = pick(timings)
+        if warm_steps > 0:
+            for b, s in enumerate(scenes):
+                eng.upload_scene(b, s)
+        eng.configure(**best)
+    finally:
+        if own:
+            eng.close()
+    res = {"signature": sig, "substeps": substeps, "warm_steps": warm_steps, "options": best,
+           "timings_ms": [{"options": c, "ms": t} for c, t in timings]}
+    _CACHE[sig] = res
+    if cache_file:
+        disk = {}
+        if os.path.exists(cache_file):
+            with open(cache_file) as f:
+                disk = json.load(f)
+        disk[sig] = res
+        os.makedirs(os.path.dirname(os.path.abspath(cache_file)), exist_ok=True)
+        with open(cache_file, "w") as f:
+            json.dump(disk, f, indent=1)
+    return res
+
+
+def clear_cache():
+    _CACHE.clear()

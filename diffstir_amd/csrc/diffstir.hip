@@ -38,8 +38,12 @@ struct ds_ctx {
     int B = 0, T = 0, K = 0, ncp = 0;
     int P = 0, nreal = 0;
     int mode = DS_KERNELS_TILED;       // requested
-    int ppt = 2;                       // particles per thread of the tiled kernels (chunk = 256 * ppt)
+    int ppt = 2;                       // rows per wavefront of the tiled scatter kernels (chunk = 4 * ppt rows)
     int ppt_sorted = 2;                // ppt the current chunk tables were built with
+    int gppt = 1;                      // particles per thread of the tiled gather kernels (256 * gppt)
+    int gppt_sorted = 1;
+    bool adj_split = false;            // adjoint P2G as two kernels (gather, stress) instead of one
+    bool packed = false;               // scatter kernels on the packed (gather) chunks with claim passes
     int alloc_mode = -1;               // mode whose buffers are allocated
     std::vector<int> nl, ns, nc;
     std::vector<Group> groups;
@@ -96,11 +100,14 @@ struct ds_ctx {
     Chunk* d_chunks = nullptr;    // ncp x maxch
     int* d_rows = nullptr;        // ncp x (P + 1) row table: first slot of every row
     GChunk* d_gchunks = nullptr;  // ncp x maxgch chunks of the gather-only kernels
-    int* d_bgch = nullptr;        // gather chunks per bucket
-    int* d_gstart = nullptr;      // first gather chunk of each bucket
+    int* d_bgch = nullptr;        // ncp x NBK gather chunks per bucket
+    int* d_gstart = nullptr;      // ncp x NBK first gather chunk of each bucket
     int* d_cov = nullptr;         // ncp x B x NB^3 node blocks reached by a chunk tile
     int* d_fbflag = nullptr;      // B x NB^3 node blocks marked by a fall-back scatter this step
     double* d_j = nullptr;        // P: J of each liquid particle for the next P2G
+    double* d_seg_mj = nullptr;   // K x 10 x P: J and its SVD backward factor per step of a segment
+    float* d_gadj = nullptr;      // 18 x P: gathered quantities of the adjoint P2G
+    double* d_thpart = nullptr;   // maxgch x 4: per-chunk parameter-gradient partials of a segment
     int* d_brows = nullptr;       // rows per bucket
     int* d_rstart = nullptr;      // first row of each bucket
     int* d_src = nullptr;         // ncp x P
@@ -322,6 +329,9 @@ void free_mode(ds_ctx* c)
     dfree(c->d_cov);
     dfree(c->d_fbflag);
     dfree(c->d_j);
+    dfree(c->d_seg_mj);
+    dfree(c->d_gadj);
+    dfree(c->d_thpart);
     dfree(c->d_brows);
     dfree(c->d_rstart);
     dfree(c->d_src);
@@ -404,12 +414,15 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_bstart, c->NBK);
         MALLOC(c->d_cstart, (size_t)c->NBK * c->ncp);
         MALLOC(c->d_tot, 4);
-        MALLOC(c->d_bgch, c->NBK);
-        MALLOC(c->d_gstart, c->NBK);
+        MALLOC(c->d_bgch, (size_t)c->NBK * c->ncp);
+        MALLOC(c->d_gstart, (size_t)c->NBK * c->ncp);
         MALLOC(c->d_gchunks, (size_t)c->maxgch * c->ncp);
         MALLOC(c->d_cov, (size_t)c->B * c->NB * c->NB * c->NB * c->ncp);
         MALLOC(c->d_fbflag, (size_t)c->B * c->NB * c->NB * c->NB);
         MALLOC(c->d_j, c->P);
+        MALLOC(c->d_seg_mj, (size_t)10 * c->P * c->K);
+        MALLOC(c->d_gadj, (size_t)18 * c->P);
+        MALLOC(c->d_thpart, (size_t)4 * c->maxgch);
         MALLOC(c->d_brows, c->NBK);
         MALLOC(c->d_rstart, c->NBK);
         MALLOC(c->d_rows, (size_t)(c->P + 1) * c->ncp);
@@ -506,23 +519,26 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     int* src = c->d_src + (size_t)cp * c->P;
     int* bnch = c->d_bnch + (size_t)cp * c->NBK;
     int* cstart = c->d_cstart + (size_t)cp * c->NBK;
+    int* bgch = c->d_bgch + (size_t)cp * c->NBK;
+    int* gstart = c->d_gstart + (size_t)cp * c->NBK;
     fill_i32(c, c->d_cnt, (long)c->NBK * 64, 0);
     hipLaunchKernelGGL(k_sort_count, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, c->d_meta,
                        c->d_groups, c->d_key, c->d_rank, c->d_cnt, c->dsc, c->NB, c->P);
     const int rpc = WAVES * c->ppt;   // rows per chunk
-    const int gch = WG;               // particles per gather chunk
+    const int gch = WG * c->gppt;     // particles per gather chunk
     c->ppt_sorted = c->ppt;
+    c->gppt_sorted = c->gppt;
     int* rows = c->d_rows + (size_t)cp * (c->P + 1);
     GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
     hipLaunchKernelGGL(k_bucket_totals, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bcount, c->d_brows, bnch, c->d_bgch, c->NBK, rpc, gch);
-    Scan4 q{{c->d_bcount, bnch, c->d_brows, c->d_bgch}, {c->d_bstart, cstart, c->d_rstart, c->d_gstart}};
+                       c->d_celloff, c->d_bcount, c->d_brows, bnch, bgch, c->NBK, rpc, gch);
+    Scan4 q{{c->d_bcount, bnch, c->d_brows, bgch}, {c->d_bstart, cstart, c->d_rstart, gstart}};
     hipLaunchKernelGGL(k_scan4, dim3(1), dim3(1024), 0, c->stream, q, c->NBK, c->d_tot);
     const long nbl = (long)c->B * c->NB * c->NB * c->NB;
     int* cov = c->d_cov + (size_t)cp * nbl;
     fill_i32(c, cov, nbl, 0);
     hipLaunchKernelGGL(k_bucket_layout, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
-                       c->d_celloff, c->d_bstart, c->d_rstart, cstart, c->d_gstart, c->d_rrpos, rows, chunks, gchunks,
+                       c->d_celloff, c->d_bstart, c->d_rstart, cstart, gstart, c->d_rrpos, rows, chunks, gchunks,
                        cov, c->NB, c->NBK, rpc, gch);
     fill_i32(c, c->d_meta2, c->P, -1);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
@@ -538,8 +554,9 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
                                     std::to_string(c->nreal) + ")");
     c->nch_h[cp] = tot[1];
     c->ngch_h[cp] = tot[3];
-    if (tot[1] > c->staging_cap) {
-        const long cap = (long)tot[1] + tot[1] / 4 + 16;
+    const long need = std::max(tot[1], tot[3]);   // staging slots of either chunk set
+    if (need > c->staging_cap) {
+        const long cap = need + need / 4 + 16;
         dfree(c->d_staging);
         c->staging_cap = 0;
         RCHK(dalloc(c, &c->d_staging, (size_t)cap * TNODES));
@@ -554,19 +571,32 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
         else if ((ppt) == 2) hipLaunchKernelGGL(kern<2>, __VA_ARGS__);                           \
         else hipLaunchKernelGGL(kern<4>, __VA_ARGS__);                                           \
     } while (0)
+#define DS_PPT_DISPATCH2(ppt, kern, b, ...)                                                      \
+    do {                                                                                         \
+        if ((ppt) == 1) hipLaunchKernelGGL((kern<1, b>), __VA_ARGS__);                           \
+        else if ((ppt) == 2) hipLaunchKernelGGL((kern<2, b>), __VA_ARGS__);                      \
+        else hipLaunchKernelGGL((kern<4, b>), __VA_ARGS__);                                      \
+    } while (0)
 
 // P2G of a checkpoint segment's chunks into the staging slots, then the grid reduction:
 // velocity into d_vgrid and, when pout is given, momentum/mass into pout.
 void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* pout, float* aout)
 {
-    const int nch = c->nch_h[cp];
-    const int* bnch = c->d_bnch + (size_t)cp * c->NBK;
-    const int* cstart = c->d_cstart + (size_t)cp * c->NBK;
+    // the scatter's chunk set: rows of distinct stencil bases, or packed gather chunks
+    const int nch = c->packed ? c->ngch_h[cp] : c->nch_h[cp];
+    const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
+    const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
     if (nch > 0) {
         KernelTimer kt(c, DS_K_P2G);
-        DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_tiled, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
-                        c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1), nch, c->d_params,
-                        c->d_staging, c->d_pfb, c->d_fbflag, c->NB, c->d_j, c->dsc, c->P, c->d_flag);
+        if (c->packed)
+            DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_tiled, GChunk, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
+                             c->d_gchunks + (size_t)cp * c->maxgch, nullptr, nch, c->d_params, c->d_staging, c->d_pfb,
+                             c->d_fbflag, c->NB, c->d_j, c->dsc, c->P, c->d_flag);
+        else
+            DS_PPT_DISPATCH2(c->ppt_sorted, k_p2g_tiled, Chunk, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
+                             c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1), nch,
+                             c->d_params, c->d_staging, c->d_pfb, c->d_fbflag, c->NB, c->d_j, c->dsc, c->P,
+                             c->d_flag);
     }
     const int nb3 = c->NB * c->NB * c->NB;
     KernelTimer kt(c, DS_K_GRID_REDUCE);
@@ -575,23 +605,34 @@ void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* p
                        c->d_pfb, c->d_fbflag, pout, c->d_vgrid, c->dsc, c->NB);
 }
 
-void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool record)
+void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool record, double* mj = nullptr)
 {
     const int ngc = c->ngch_h[cp];
     if (ngc == 0) return;
-    KernelTimer kt(c, DS_K_G2P);
-    DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_tiled, dim3(ngc), dim3(WG), 0, c->stream, in, out,
-                    c->d_gchunks + (size_t)cp * c->maxgch, ngc, c->d_vgrid, c->d_traj, c->T, f,
-                    record ? c->d_dvsum : nullptr, c->d_j, c->dsc, c->P, c->d_flag);
+    KernelTimer kt(c, mj ? DS_K_G2P_FACTOR : DS_K_G2P);
+    const GChunk* gch = c->d_gchunks + (size_t)cp * c->maxgch;
+    double* dv = record ? c->d_dvsum : nullptr;
+    if (mj)
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, true, dim3(ngc), dim3(WG), 0, c->stream, in, out, gch, ngc,
+                         c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P, c->d_flag);
+    else
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, false, dim3(ngc), dim3(WG), 0, c->stream, in, out, gch, ngc,
+                         c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P, c->d_flag);
 }
 
-// J of every liquid particle of the checkpoint state `st` into d_j, for the segment's first P2G.
-void liquid_j_tiled(ds_ctx* c, const float* st, int cp)
+// J of every liquid particle of the checkpoint state `st` into d_j, for the segment's first P2G
+// (and, given mj, J and its SVD backward factor into mj).
+void liquid_j_tiled(ds_ctx* c, const float* st, int cp, double* mj = nullptr)
 {
     const int ngc = c->ngch_h[cp];
     if (ngc == 0) return;
-    DS_PPT_DISPATCH(c->ppt_sorted, k_liquid_j, dim3(ngc), dim3(WG), 0, c->stream, st,
-                    c->d_gchunks + (size_t)cp * c->maxgch, ngc, c->d_j, c->dsc, c->P);
+    const GChunk* gch = c->d_gchunks + (size_t)cp * c->maxgch;
+    if (mj)
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, true, dim3(ngc), dim3(WG), 0, c->stream, st, gch, ngc, c->d_j, mj,
+                         c->dsc, c->P);
+    else
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, false, dim3(ngc), dim3(WG), 0, c->stream, st, gch, ngc, c->d_j, mj,
+                         c->dsc, c->P);
 }
 
 int forward_tiled(ds_ctx* c, int nsteps)
@@ -632,29 +673,36 @@ int backward_tiled(ds_ctx* c)
         const int* rows = c->d_rows + (size_t)cp * (c->P + 1);
         const int ngc = c->ngch_h[cp];
         const GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
-        const int* bnch = c->d_bnch + (size_t)cp * c->NBK;
-        const int* cstart = c->d_cstart + (size_t)cp * c->NBK;
+        const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
+        const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
         const float* ck = c->d_ckpt + (size_t)cp * SF;
         auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : c->d_seg + (size_t)k * SF; };
         auto grid_of = [&](int k) -> float4* { return c->d_seg_grid + (size_t)k * G; };
         auto aff_of = [&](int k) -> float* { return c->d_seg_aff + (size_t)k * 9 * c->P; };
+        auto mj_of = [&](int k) -> double* { return c->d_seg_mj + (size_t)k * 10 * c->P; };
         // recompute the segment, keeping each step's state and momentum/mass grid
-        liquid_j_tiled(c, ck, cp);
+        fill_zero_f64(c, c->d_thpart, 4L * std::max(ngc, 1));
+        liquid_j_tiled(c, ck, cp, mj_of(0));
         for (int k = 0; k < L; k++) {
             const int s = s0 + k;
             c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
             p2g_reduce_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k), aff_of(k));
-            if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, s, false);
+            if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, s, false, mj_of(k + 1));
         }
         // adjoint, last step first
         for (int k = L - 1; k >= 0; k--) {
             const int s = s0 + k;
             c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
-            if (nch > 0) {
+            if (c->packed && ngc > 0) {
                 KernelTimer kt(c, DS_K_G2P_ADJ);
-                DS_PPT_DISPATCH(c->ppt_sorted, k_g2p_adj_tiled, dim3(nch), dim3(WG), 0, c->stream, st_of(k), c->d_adj,
-                                chunks, rows, nch, grid_of(k), c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb,
-                                c->d_fbflag, c->NB, c->d_xbar, c->dsc, c->P);
+                DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_adj_tiled, GChunk, dim3(ngc), dim3(WG), 0, c->stream, st_of(k),
+                                 c->d_adj, gchunks, nullptr, ngc, c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb,
+                                 c->d_fbflag, c->NB, c->dsc, c->P);
+            } else if (!c->packed && nch > 0) {
+                KernelTimer kt(c, DS_K_G2P_ADJ);
+                DS_PPT_DISPATCH2(c->ppt_sorted, k_g2p_adj_tiled, Chunk, dim3(nch), dim3(WG), 0, c->stream, st_of(k),
+                                 c->d_adj, chunks, rows, nch, c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb,
+                                 c->d_fbflag, c->NB, c->dsc, c->P);
             }
             {
                 KernelTimer kt(c, DS_K_VBAR_REDUCE);
@@ -662,11 +710,26 @@ int backward_tiled(ds_ctx* c)
                                    c->d_cov + (size_t)cp * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2,
                                    c->dsc, c->NB);
             }
-            KernelTimer kt(c, DS_K_P2G_ADJ);
-            if (ngc > 0)
-                DS_PPT_DISPATCH(c->ppt_sorted, k_p2g_adj_tiled, dim3(ngc), dim3(WG), 0, c->stream, st_of(k), aff_of(k),
-                                c->d_adj, c->d_xbar, gchunks, ngc, c->d_params, c->d_pbar2, c->d_thbar, c->dsc, c->P);
+            if (ngc > 0 && c->adj_split) {
+                {
+                    KernelTimer kt(c, DS_K_P2G_ADJ);
+                    DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, false, dim3(ngc), dim3(WG), 0, c->stream,
+                                     st_of(k), aff_of(k), c->d_adj, gchunks, ngc, grid_of(k), c->d_pbar2, c->d_gadj,
+                                     c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
+                }
+                KernelTimer kt(c, DS_K_P2G_ADJ_STRESS);
+                DS_PPT_DISPATCH(c->gppt_sorted, k_p2g_adj_stress, dim3(ngc), dim3(WG), 0, c->stream, st_of(k),
+                                c->d_adj, gchunks, ngc, c->d_params, c->d_gadj, mj_of(k), c->d_thpart, c->dsc, c->P);
+            } else if (ngc > 0) {
+                KernelTimer kt(c, DS_K_P2G_ADJ);
+                DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, true, dim3(ngc), dim3(WG), 0, c->stream, st_of(k),
+                                 aff_of(k), c->d_adj, gchunks, ngc, grid_of(k), c->d_pbar2, c->d_gadj, c->d_params,
+                                 mj_of(k), c->d_thpart, c->dsc, c->P);
+            }
         }
+        if (ngc > 0)
+            hipLaunchKernelGGL(k_thpart_reduce, dim3((ngc + 255) / 256), dim3(256), 0, c->stream, gchunks, ngc,
+                               c->d_thpart, c->d_thbar);
         if (cp > 0) {
             hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
                                c->d_src + (size_t)cp * c->P, c->nreal, c->P);
@@ -830,11 +893,24 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         c->kt_every = value;
         return DS_OK;
     }
-    if (option == DS_OPT_PARTICLES_PER_THREAD) {
+    if (option == DS_OPT_PARTICLES_PER_THREAD || option == DS_OPT_GATHER_PARTICLES_PER_THREAD) {
         if (value != 1 && value != 2 && value != 4)
-            return fail(c, DS_EINVAL, "DS_OPT_PARTICLES_PER_THREAD takes 1, 2 or 4");
-        if (value != c->ppt) c->reset_done = false;
-        c->ppt = value;
+            return fail(c, DS_EINVAL, "particles per thread must be 1, 2 or 4");
+        int& dst = option == DS_OPT_PARTICLES_PER_THREAD ? c->ppt : c->gppt;
+        if (value != dst) c->reset_done = false;
+        dst = value;
+        return DS_OK;
+    }
+    if (option == DS_OPT_SCATTER_CHUNKS) {
+        if (value != DS_SCATTER_ROWS && value != DS_SCATTER_PACKED)
+            return fail(c, DS_EINVAL, "DS_OPT_SCATTER_CHUNKS takes DS_SCATTER_ROWS or DS_SCATTER_PACKED");
+        c->packed = value == DS_SCATTER_PACKED;
+        return DS_OK;
+    }
+    if (option == DS_OPT_ADJOINT_P2G) {
+        if (value != DS_ADJ_P2G_FUSED && value != DS_ADJ_P2G_SPLIT)
+            return fail(c, DS_EINVAL, "DS_OPT_ADJOINT_P2G takes DS_ADJ_P2G_FUSED or DS_ADJ_P2G_SPLIT");
+        c->adj_split = value == DS_ADJ_P2G_SPLIT;
         return DS_OK;
     }
     return fail(c, DS_EINVAL, "unknown option");

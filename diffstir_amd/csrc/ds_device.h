@@ -379,6 +379,14 @@ __device__ __forceinline__ void svd3_backward_sigma(const double U[9], const dou
                           + U[i * 3 + 2] * gS[2] * V[j * 3 + 2];
 }
 
+// The SVD backward of J = S0 S1 S2 up to the factor dL/dJ: with gS = dL/dJ (S1 S2, S0 S2, S0 S1),
+// U diag(gS) V^T = dL/dJ * M, M = U diag(S1 S2, S0 S2, S0 S1) V^T.
+__device__ __forceinline__ void svd3_j_factor(const double U[9], const double S[3], const double V[9], double M[9])
+{
+    const double g[3] = {S[1] * S[2], S[0] * S[2], S[0] * S[1]};
+    svd3_backward_sigma(U, V, g, M);
+}
+
 // ---------------------------------------------------------------------------- liquid stress
 
 // Everything P2G needs from one liquid particle (simulation_functions.py:111-153):
@@ -414,6 +422,17 @@ __device__ __forceinline__ double liquid_j(const float Fn[9])
 #pragma unroll
     for (int i = 0; i < 9; i++) Fd[i] = (double)Fn[i];
     svd3_values(Fd, S);
+    return S[0] * S[1] * S[2];
+}
+
+// J and the SVD backward factor M of J (svd3_j_factor) from the full FP64 SVD of F_tmp.
+__device__ __forceinline__ double liquid_j_factor(const float Fn[9], double M[9])
+{
+    double Fd[9], U[9], S[3], V[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Fd[i] = (double)Fn[i];
+    svd3(Fd, U, S, V);
+    svd3_j_factor(U, S, V, M);
     return S[0] * S[1] * S[2];
 }
 

@@ -285,12 +285,14 @@ __device__ __forceinline__ void p2g_adj_gather(const DevScene& sc, const Stencil
 // Adjoint of P2G for one liquid particle, stress half: from Ab = dL/dA, back through
 // A = alpha S + m C, S = eta(|C|) sym(C) + I (k/2)(J - 1/J - 1) J, J = S0 S1 S2 of the FP64 SVD
 // of F_tmp (simulation_functions.py:111-153) and F_tmp = (I + dt C) F.  L holds the particle's
-// P2G quantities (liquid_p2g); Fnb_in is the adjoint of F_tmp from the next step's F and
-// xb_partial the G2P part of dL/dx.  Writes the adjoints of x, v, C, F and adds dL/dparams into tb.
+// P2G quantities (eta, gamma, J); M is the SVD backward factor of J (svd3_j_factor); Fnb_in is
+// the adjoint of F_tmp from the next step's F and xb_partial the G2P part of dL/dx.  Writes the
+// adjoints of x, v, C, F and adds dL/dparams into tb.
 __device__ __forceinline__ void p2g_adj_stress(const DevScene& sc, const InstParams& ip, const ParticleIn& q,
-                                               const LiquidP2G& L, const float ub[3], const float Ab[9],
-                                               const float fxb[3], const float Fnb_in[9], const float xb_partial[3],
-                                               float xo[3], float vo[3], float Co[9], float Fo[9], double tb[4])
+                                               const LiquidP2G& L, const double M[9], const float ub[3],
+                                               const float Ab[9], const float fxb[3], const float Fnb_in[9],
+                                               const float xb_partial[3], float xo[3], float vo[3], float Co[9],
+                                               float Fo[9], double tb[4])
 {
     const float m = sc.m_liq;
     const float* C = q.C;
@@ -325,13 +327,10 @@ __device__ __forceinline__ void p2g_adj_stress(const DevScene& sc, const InstPar
 #pragma unroll
         for (int a = 0; a < 9; a++) Cb[a] += gb * C[a] * inr;
     }
-    // J = S0 S1 S2 from the FP64 SVD of F_tmp: dJ/dS_i is the product of the other two
-    double gS[3] = {Jb * L.S[1] * L.S[2], Jb * L.S[0] * L.S[2], Jb * L.S[0] * L.S[1]};
-    double gFn[9];
-    svd3_backward_sigma(L.U, L.V, gS, gFn);
+    // J = S0 S1 S2 from the FP64 SVD of F_tmp: dL/dF_tmp = U diag(dL/dS) V^T = Jb M
     float Fnb[9];
 #pragma unroll
-    for (int a = 0; a < 9; a++) Fnb[a] = Fnb_in[a] + (float)gFn[a];
+    for (int a = 0; a < 9; a++) Fnb[a] = Fnb_in[a] + (float)(Jb * M[a]);
     // F_tmp = (I + dt C) F
 #pragma unroll
     for (int a = 0; a < 3; a++)
@@ -364,10 +363,12 @@ __device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstPar
 {
     LiquidP2G L;
     liquid_p2g<true>(sc, ip.th, ip.kb, q.C, q.F, L);
+    double M[9];
+    svd3_j_factor(L.U, L.S, L.V, M);
     const float u[3] = {q.v[0], q.v[1] + sc.gdt, q.v[2]};
     float ub[3], Ab[9], fxb[3];
     p2g_adj_gather(sc, s, L.A, u, acc, ub, Ab, fxb);
-    p2g_adj_stress(sc, ip, q, L, ub, Ab, fxb, Fnb_in, xb_partial, xo, vo, Co, Fo, tb);
+    p2g_adj_stress(sc, ip, q, L, M, ub, Ab, fxb, Fnb_in, xb_partial, xo, vo, Co, Fo, tb);
 }
 
 // Adjoint of P2G for a stick or container particle: only C carries an adjoint

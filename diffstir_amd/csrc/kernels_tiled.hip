@@ -71,6 +71,23 @@ __device__ __forceinline__ bool chunk_slot(const Chunk& ch, const int* __restric
     return p < re;
 }
 
+// The particle of (iteration r, wavefront, lane) in a scatter kernel: row r * WAVES + wave of a
+// row chunk, or slot r * WG + wave * 64 + lane of a packed (gather) chunk, whose lanes may share
+// stencil bases (the claim protocol then serves them in successive passes).
+__device__ __forceinline__ bool scatter_slot(const Chunk& ch, const int* __restrict__ rows, int r, int wave, int lane,
+                                             int& p)
+{
+    return chunk_slot(ch, rows, r * WAVES + wave, lane, p);
+}
+
+__device__ __forceinline__ bool scatter_slot(const GChunk& ch, const int* __restrict__, int r, int wave, int lane,
+                                             int& p)
+{
+    const int idx = r * WG + wave * 64 + lane;
+    p = ch.start + idx;
+    return idx < ch.count;
+}
+
 __device__ __forceinline__ int tile_id(int lx, int ly, int lz) { return lx * TSX + ly * TSY + lz; }
 
 __device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0, int z0)
@@ -208,9 +225,9 @@ __device__ __forceinline__ void store_staging(const float* wt, int wstride, int 
 
 // ---------------------------------------------------------------------------- forward
 
-template <int PPT>
+template <int PPT, class CH>
 __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
-                                                  float* __restrict__ aout, const Chunk* __restrict__ chunks,
+                                                  float* __restrict__ aout, const CH* __restrict__ chunks,
                                                   const int* __restrict__ rows,
                                                   int nchunks,
                                                   const double* __restrict__ params, float4* __restrict__ staging,
@@ -226,13 +243,13 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     unsigned char* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const Chunk ch = chunks[c];
+        const CH ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const bool liq = ch.species == LIQUID;
         // the first row's particles are loaded before the tile is cleared, and each next row's
         // while the current one scatters
         int p = 0;
-        bool valid = chunk_slot(ch, rows, wave, lane, p);
+        bool valid = scatter_slot(ch, rows, 0, wave, lane, p);
         ParticleIn q;
         double J = 1.0;
         if (valid) {
@@ -267,7 +284,7 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
                 }
             }
             int pn = 0;
-            const bool vn = r + 1 < PPT && chunk_slot(ch, rows, (r + 1) * WAVES + wave, lane, pn);
+            const bool vn = r + 1 < PPT && scatter_slot(ch, rows, r + 1, wave, lane, pn);
             if (vn) {
                 load_particle_t(in, LP, pn, liq, q);
                 if (liq) J = jin[pn];
@@ -391,12 +408,15 @@ __global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ s
     vout[gid] = make_float4(v[0], v[1], v[2], 0.0f);
 }
 
-template <int PPT>
+// kFactor (the recompute of the adjoint): the full FP64 SVD, storing J and its SVD backward
+// factor M for the adjoint P2G of the next step in mj ([10][P]: M row-major, then J).
+template <int PPT, bool kFactor>
 __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
                                                   const GChunk* __restrict__ chunks, int nchunks,
                                                   const float4* __restrict__ vgrid, const float* __restrict__ traj,
                                                   int T, int f, double* __restrict__ dvsum, double* __restrict__ jout,
-                                                  DevScene sc, int P, unsigned* __restrict__ flag)
+                                                  double* __restrict__ mj, DevScene sc, int P,
+                                                  unsigned* __restrict__ flag)
 {
     __shared__ float tv[3 * TILE];
     __shared__ double red[WAVES][3];
@@ -454,7 +474,16 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
                     // J of the next step's P2G: F_tmp = (I + dt C') F'
                     float Fn[9];
                     f_tmp(sc, nC, Fo, Fn);
-                    jout[p] = liquid_j(Fn);
+                    if (kFactor) {
+                        double M[9];
+                        const double J = liquid_j_factor(Fn, M);
+#pragma unroll
+                        for (int a = 0; a < 9; a++) mj[a * LP + p] = M[a];
+                        mj[9 * LP + p] = J;
+                        jout[p] = J;
+                    } else {
+                        jout[p] = liquid_j(Fn);
+                    }
                 }
             } else if (ch.species == STICK) {
 #pragma unroll
@@ -494,10 +523,12 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
 }
 
 // J of every liquid particle of a state (F_tmp = (I + dt C) F, values-only FP64 SVD), for the
-// first P2G after a checkpoint; later steps get J from the previous step's G2P.
-template <int PPT>
+// first P2G after a checkpoint; later steps get J from the previous step's G2P.  kFactor: the
+// full SVD, also storing J and M in mj as k_g2p_tiled does.
+template <int PPT, bool kFactor>
 __global__ void __launch_bounds__(WG) k_liquid_j(const float* __restrict__ st, const GChunk* __restrict__ chunks,
-                                                 int nchunks, double* __restrict__ jout, DevScene sc, int P)
+                                                 int nchunks, double* __restrict__ jout, double* __restrict__ mj,
+                                                 DevScene sc, int P)
 {
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -515,7 +546,16 @@ __global__ void __launch_bounds__(WG) k_liquid_j(const float* __restrict__ st, c
                 F[a] = st[(SF + a) * LP + p];
             }
             f_tmp(sc, C, F, Fn);
-            jout[p] = liquid_j(Fn);
+            if (kFactor) {
+                double M[9];
+                const double J = liquid_j_factor(Fn, M);
+#pragma unroll
+                for (int a = 0; a < 9; a++) mj[a * LP + p] = M[a];
+                mj[9 * LP + p] = J;
+                jout[p] = J;
+            } else {
+                jout[p] = liquid_j(Fn);
+            }
         }
     }
 }
@@ -542,17 +582,13 @@ __device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __re
     }
 }
 
-template <int PPT>
+template <int PPT, class CH>
 __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ st, const float* __restrict__ adj,
-                                                      const Chunk* __restrict__ chunks, const int* __restrict__ rows,
-                                                      int nchunks,
-                                                      const float4* __restrict__ grid,
-                                                      const double* __restrict__ dvbar, int T, int f,
+                                                      const CH* __restrict__ chunks, const int* __restrict__ rows,
+                                                      int nchunks, const double* __restrict__ dvbar, int T, int f,
                                                       float4* __restrict__ staging, float4* __restrict__ fallback,
-                                                      int* __restrict__ fbflag, int NB, float* __restrict__ xbar,
-                                                      DevScene sc, int P)
+                                                      int* __restrict__ fbflag, int NB, DevScene sc, int P)
 {
-    __shared__ float tv[3 * TILE];
     __shared__ __align__(16) float wt[WAVES * 3 * TILE];
     __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
@@ -561,58 +597,53 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     unsigned char* ow = own + wave * TNODES;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const Chunk ch = chunks[c];
+        const CH ch = chunks[c];
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
-        const float4* gi = grid + (long)ch.inst * nn;
         const bool liq = ch.species == LIQUID;
-        if (liq) load_velocity_tile(tv, gi, sc, x0, y0, z0);
-        zero_lds(wt, WAVES * 3 * TILE);
-        __syncthreads();
         float vstick[3] = {0.0f, 0.0f, 0.0f};
         if (ch.species == STICK) {
             const double* d = dvbar + ((long)ch.inst * T + f + 1) * 3;
 #pragma unroll
             for (int a = 0; a < 3; a++) vstick[a] = -(float)d[a];
         }
+        // the first row's particles are loaded before the tile is cleared, and each next row's
+        // while the current one scatters
+        auto load = [&](int q, float x[3], float Cb[9], float vt[3]) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + q];
+#pragma unroll
+            for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + q];
+#pragma unroll
+            for (int a = 0; a < 3; a++) vt[a] = liq ? adj[(SV + a) * LP + q] + sc.dt * adj[(SX + a) * LP + q] : vstick[a];
+        };
+        int p = 0;
+        bool valid = scatter_slot(ch, rows, 0, wave, lane, p);
+        float x[3], Cb[9], vt[3];
+        if (valid) load(p, x, Cb, vt);
+        zero_lds(wt, WAVES * 3 * TILE);
         WaveScatter3 acc{wt + wave * 3 * TILE, x0, y0, z0};
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
-            int p = 0;
-            const bool valid = chunk_slot(ch, rows, r * WAVES + wave, lane, p);
             Stencil s;
-            float Cb[9], vt[3] = {vstick[0], vstick[1], vstick[2]};
             bool it = false;
+            float Cc[9], vc[3];
             if (valid) {
-                float x[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
                 stencil(sc, x[0], x[1], x[2], s);
                 it = stencil_in_tile(s, x0, y0, z0);
 #pragma unroll
-                for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
-                if (liq) {
-                    float xa[3], xb[3];
+                for (int a = 0; a < 9; a++) Cc[a] = Cb[a];
 #pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        xa[a] = adj[(SX + a) * LP + p];
-                        vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
-                    }
-                    if (it) {
-                        TileV gv{tv, x0, y0, z0};
-                        g2p_adj_xgrad(sc, s, vt, Cb, gv, xb);
-                    } else {
-                        GatherVfromP gv{gi, sc};
-                        g2p_adj_xgrad<true>(sc, s, vt, Cb, gv, xb);
-                    }
-#pragma unroll
-                    for (int a = 0; a < 3; a++) xbar[a * LP + p] = xa[a] + xb[a];
-                }
+                for (int a = 0; a < 3; a++) vc[a] = vt[a];
                 if (!it) {
                     mark_fallback_blocks(fbflag, ch.inst, s, NB);
                     ScatterGlobal3 g{fallback + (long)ch.inst * nn, n};
-                    g2p_adj_scatter(sc, s, vt, Cb, g);
+                    g2p_adj_scatter(sc, s, vc, Cc, g);
                 }
             }
+            int pn = 0;
+            const bool vn = r + 1 < PPT && scatter_slot(ch, rows, r + 1, wave, lane, pn);
+            if (vn) load(pn, x, Cb, vt);
+            if (r == 0) __syncthreads();   // every wavefront's tile is cleared
             bool pending = valid && it;
             const int key = pending ? node_key(s, x0, y0, z0) : 0;
             while (__ballot(pending)) {
@@ -620,10 +651,11 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
                 if (pending) ow[key] = (unsigned char)lane;
                 lds_order();
                 if (pending && ow[key] == lane) {
-                    g2p_adj_scatter(sc, opaque(s), vt, Cb, acc);
+                    g2p_adj_scatter(sc, opaque(s), vc, Cc, acc);
                     pending = false;
                 }
             }
+            valid = vn;
         }
         __syncthreads();
         store_staging(wt, 3 * TILE, 3, staging + (long)c * TNODES);
@@ -667,15 +699,75 @@ __global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ s
     pbar[gid] = gridop_node_adj(r, acc.x, acc.y, acc.z);
 }
 
-template <int PPT>
+// The stress half of the adjoint P2G of liquid particle p (xp, ub, Ab, fxb gathered): with the
+// J and SVD backward factor stored by the recompute pass, the adjoint of the particle's state
+// written in place of the next step's, and the parameter gradient added to tb.
+__device__ __forceinline__ void p2g_adj_stress_at(const DevScene& sc, const InstParams& ip,
+                                                  const float* __restrict__ st, float* __restrict__ adj,
+                                                  const double* __restrict__ mj, long LP, int p, const float xp[3],
+                                                  const float ub[3], const float Ab[9], const float fxb[3],
+                                                  double tb[4])
+{
+    ParticleIn q;
+#pragma unroll
+    for (int a = 0; a < 9; a++) {
+        q.C[a] = st[(SC + a) * LP + p];
+        q.F[a] = st[(SF + a) * LP + p];
+    }
+    float Fnb[9], xo[3], vo[3], Co[9], Fo[9];
+#pragma unroll
+    for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
+    double M[9];
+#pragma unroll
+    for (int a = 0; a < 9; a++) M[a] = mj[a * LP + p];
+    LiquidP2G L;
+    liquid_stress(sc, ip.th, ip.kb, q.C, mj[9 * LP + p], L);
+    p2g_adj_stress(sc, ip, q, L, M, ub, Ab, fxb, Fnb, xp, xo, vo, Co, Fo, tb);
+#pragma unroll
+    for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
+#pragma unroll
+    for (int a = 0; a < 3; a++) adj[(SV + a) * LP + p] = vo[a];
+#pragma unroll
+    for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
+#pragma unroll
+    for (int a = 0; a < 9; a++) adj[(SF + a) * LP + p] = Fo[a];
+}
+
+// Workgroup sum of the four FP64 parameter-gradient partials into the chunk's slot of thpart.
+__device__ __forceinline__ void add_chunk_partials(double (*red)[4], const double tb[4], double* __restrict__ thpart,
+                                                   int c)
+{
+    const double r0 = wave_sum(tb[0]), r1 = wave_sum(tb[1]), r2 = wave_sum(tb[2]), r3 = wave_sum(tb[3]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = r0;
+        red[w][1] = r1;
+        red[w][2] = r2;
+        red[w][3] = r3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double a2 = 0.0;
+        for (int k = 0; k < WAVES; k++) a2 += red[k][threadIdx.x];
+        thpart[4 * (long)c + threadIdx.x] += a2;
+    }
+}
+
+// Adjoint P2G, gather half: for liquid particles the G2P part of dL/dx (xp, through the gather
+// of this step's velocity) and the gather of dL/d(momentum, mass) with the stored affine (ub, Ab,
+// fxb); for stick and container particles their whole adjoint (dL/dC).  kFused: the stress half
+// follows in the same thread (p2g_adj_stress_at, partials into thpart); otherwise the gathered
+// quantities go to gout ([18][P]: xp, ub, Ab, fxb) for k_p2g_adj_stress.
+template <int PPT, bool kFused>
 __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, const float* __restrict__ ain,
-                                                      float* __restrict__ adj, const float* __restrict__ xbar,
-                                                      const GChunk* __restrict__ chunks,
-                                                      int nchunks, const double* __restrict__ params,
-                                                      const float4* __restrict__ pbar, double* __restrict__ thbar,
-                                                      DevScene sc, int P)
+                                                      float* __restrict__ adj, const GChunk* __restrict__ chunks,
+                                                      int nchunks, const float4* __restrict__ grid,
+                                                      const float4* __restrict__ pbar, float* __restrict__ gout,
+                                                      const double* __restrict__ params, const double* __restrict__ mj,
+                                                      double* __restrict__ thpart, DevScene sc, int P)
 {
     __shared__ float tq[4 * TILE];
+    __shared__ float tv[3 * TILE];
     __shared__ double red[WAVES][4];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
@@ -695,7 +787,107 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             tq[2 * TILE + id] = v.z;
             tq[3 * TILE + id] = v.w;
         }
+        const float4* gi = grid + (long)ch.inst * nn;
+        if (ch.species == LIQUID) load_velocity_tile(tv, gi, sc, x0, y0, z0);
         __syncthreads();
+        double tb[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int r = 0; r < PPT; r++) {
+            const int idx = r * WG + threadIdx.x;
+            if (idx >= ch.count) break;
+            const int p = ch.start + idx;
+            float x[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
+            Stencil s;
+            stencil(sc, x[0], x[1], x[2], s);
+            const bool it = stencil_in_tile(s, x0, y0, z0);
+            if (ch.species == LIQUID) {
+                float xp[3];
+                {
+                    // the G2P part of dL/dx (its weights and offsets depend on x): dL/dx of the
+                    // next step plus the gradient through the gather of this step's velocity
+                    float xa[3], vt[3], Cb[9];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        xa[a] = adj[(SX + a) * LP + p];
+                        vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
+                    if (it) {
+                        TileV gv{tv, x0, y0, z0};
+                        g2p_adj_xgrad(sc, s, vt, Cb, gv, xp);
+                    } else {
+                        GatherVfromP gv{gi, sc};
+                        g2p_adj_xgrad<true>(sc, s, vt, Cb, gv, xp);
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; a++) xp[a] += xa[a];
+                    if (!kFused) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) gout[a * LP + p] = xp[a];
+                    }
+                }
+                asm volatile("" ::: "memory");
+                // the gather with the affine stored by the recompute pass
+                float A[9], ub[3], Ab[9], fxb[3];
+#pragma unroll
+                for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
+                const float u[3] = {st[SV * LP + p], st[(SV + 1) * LP + p] + sc.gdt, st[(SV + 2) * LP + p]};
+                if (it) {
+                    TileP4 g{tq, x0, y0, z0};
+                    p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
+                } else {
+                    GatherPbar g{qi, n};
+                    p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
+                }
+                if (kFused) {
+                    asm volatile("" ::: "memory");
+                    p2g_adj_stress_at(sc, load_params(sc, params, ch.inst), st, adj, mj, LP, p, xp, ub, Ab, fxb,
+                                      tb);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < 3; a++) gout[(3 + a) * LP + p] = ub[a];
+#pragma unroll
+                    for (int a = 0; a < 9; a++) gout[(6 + a) * LP + p] = Ab[a];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) gout[(15 + a) * LP + p] = fxb[a];
+                }
+            } else {
+                float Co[9];
+                if (it) {
+                    TileP4 g{tq, x0, y0, z0};
+                    p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
+                } else {
+                    GatherPbar g{qi, n};
+                    p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
+                }
+#pragma unroll
+                for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
+            }
+        }
+        if (kFused && ch.species == LIQUID) add_chunk_partials(red, tb, thpart, c);
+        __syncthreads();
+    }
+}
+
+// Adjoint P2G, stress half, one thread per liquid particle: from the gathered quantities and
+// the J and SVD backward factor stored by the recompute pass, the adjoint of the particle's
+// state (written in place of the next step's) and the FP64 parameter gradient (wave, then
+// workgroup sums into the chunk's slot of thpart).
+template <int PPT>
+__global__ void __launch_bounds__(WG) k_p2g_adj_stress(const float* __restrict__ st, float* __restrict__ adj,
+                                                       const GChunk* __restrict__ chunks, int nchunks,
+                                                       const double* __restrict__ params,
+                                                       const float* __restrict__ gin, const double* __restrict__ mj,
+                                                       double* __restrict__ thpart, DevScene sc, int P)
+{
+    __shared__ double red[WAVES][4];
+    const long LP = P;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const GChunk ch = chunks[c];
+        if (ch.species != LIQUID) continue;
         double tb[4] = {0.0, 0.0, 0.0, 0.0};
         const InstParams ip = load_params(sc, params, ch.inst);
 #pragma unroll 1
@@ -703,78 +895,57 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             const int idx = r * WG + threadIdx.x;
             if (idx >= ch.count) break;
             const int p = ch.start + idx;
-            if (ch.species == LIQUID) {
-                ParticleIn q;
-                load_particle_t(st, LP, p, true, q);
-                Stencil s;
-                stencil(sc, q.x[0], q.x[1], q.x[2], s);
-                float Fnb[9], xp[3], xo[3], vo[3], Co[9], Fo[9];
-                // gather with the affine stored by the recompute pass, then the FP64 SVD and the
-                // stress adjoint.  The empty asm ties C to the gather's results so the compiler
-                // cannot hoist the (independent) SVD into the gather and keep both live.
-                float A[9], ub[3], Ab[9], fxb[3];
+            float xp[3], ub[3], Ab[9], fxb[3];
 #pragma unroll
-                for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
-                const float u[3] = {q.v[0], q.v[1] + sc.gdt, q.v[2]};
-                if (stencil_in_tile(s, x0, y0, z0)) {
-                    TileP4 g{tq, x0, y0, z0};
-                    p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
-                } else {
-                    GatherPbar g{qi, n};
-                    p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
-                }
-#pragma unroll
-                for (int a = 0; a < 9; a++) asm volatile("" : "+v"(q.C[a]) : "v"(Ab[a]), "v"(ub[a % 3]), "v"(fxb[a % 3]));
-#pragma unroll
-                for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
-#pragma unroll
-                for (int a = 0; a < 3; a++) xp[a] = xbar[a * LP + p];
-                LiquidP2G L;
-                liquid_p2g<true>(sc, ip.th, ip.kb, q.C, q.F, L);
-                p2g_adj_stress(sc, ip, q, L, ub, Ab, fxb, Fnb, xp, xo, vo, Co, Fo, tb);
-#pragma unroll
-                for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
-#pragma unroll
-                for (int a = 0; a < 3; a++) adj[(SV + a) * LP + p] = vo[a];
-#pragma unroll
-                for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
-#pragma unroll
-                for (int a = 0; a < 9; a++) adj[(SF + a) * LP + p] = Fo[a];
-            } else {
-                float x[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
-                Stencil s;
-                stencil(sc, x[0], x[1], x[2], s);
-                float Co[9];
-                if (stencil_in_tile(s, x0, y0, z0)) {
-                    TileP4 g{tq, x0, y0, z0};
-                    p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
-                } else {
-                    GatherPbar g{qi, n};
-                    p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
-                }
-#pragma unroll
-                for (int a = 0; a < 9; a++) adj[(SC + a) * LP + p] = Co[a];
+            for (int a = 0; a < 3; a++) {
+                xp[a] = gin[a * LP + p];
+                ub[a] = gin[(3 + a) * LP + p];
+                fxb[a] = gin[(15 + a) * LP + p];
             }
+#pragma unroll
+            for (int a = 0; a < 9; a++) Ab[a] = gin[(6 + a) * LP + p];
+            p2g_adj_stress_at(sc, ip, st, adj, mj, LP, p, xp, ub, Ab, fxb, tb);
         }
-        if (ch.species == LIQUID) {
-            const double r0 = wave_sum(tb[0]), r1 = wave_sum(tb[1]), r2 = wave_sum(tb[2]), r3 = wave_sum(tb[3]);
-            const int w = threadIdx.x >> 6;
-            if ((threadIdx.x & 63) == 0) {
-                red[w][0] = r0;
-                red[w][1] = r1;
-                red[w][2] = r2;
-                red[w][3] = r3;
-            }
-            __syncthreads();
-            if (threadIdx.x < 4) {
-                double a2 = 0.0;
-                for (int k = 0; k < WAVES; k++) a2 += red[k][threadIdx.x];
-                atomicAdd(thbar + 4 * ch.inst + threadIdx.x, a2);
-            }
+        add_chunk_partials(red, tb, thpart, c);
+        __syncthreads();
+    }
+}
+
+// The per-chunk parameter-gradient partials of a segment into the per-instance gradient: a
+// workgroup sums the partials of its chunks with one atomic per parameter when they all belong
+// to one instance (the usual case: chunks are ordered by instance), else one per chunk.
+__global__ void __launch_bounds__(256) k_thpart_reduce(const GChunk* __restrict__ chunks, int nchunks,
+                                                       const double* __restrict__ thpart, double* __restrict__ thbar)
+{
+    __shared__ double red[WAVES][4];
+    __shared__ int inst0;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int last = min(nchunks, (int)(blockIdx.x + 1) * (int)blockDim.x) - 1;
+    if (threadIdx.x == 0) inst0 = chunks[blockIdx.x * blockDim.x].inst;
+    __syncthreads();
+    const bool ok = c < nchunks;
+    const int inst = ok ? chunks[c].inst : inst0;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (ok) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = thpart[4 * (long)c + k];
+    }
+    if (__syncthreads_and(inst == inst0) && chunks[last].inst == inst0) {
+        const int w = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double r = wave_sum(v[k]);
+            if ((threadIdx.x & 63) == 0) red[w][k] = r;
         }
         __syncthreads();
+        if (threadIdx.x < 4) {
+            double a = 0.0;
+            for (int k = 0; k < WAVES; k++) a += red[k][threadIdx.x];
+            atomicAdd(thbar + 4 * inst0 + threadIdx.x, a);
+        }
+    } else if (ok) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) atomicAdd(thbar + 4 * inst + k, v[k]);
     }
 }
 

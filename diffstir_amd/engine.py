@@ -44,8 +44,14 @@ class Engine:
 
     KERNELS = {"tiled": 0, "per_particle": 1}
 
+    # execution options of the tiled kernels (ds_set_option ids)
+    OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER = 1, 2, 3, 4, 5, 6
+    ADJ_P2G = {"fused": 0, "split": 1}
+    SCATTER = {"rows": 0, "packed": 1}
+
     def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None,
-                 kernels: str = "tiled", particles_per_thread: int = 2, kernel_timing_every: int = 0):
+                 kernels: str = "tiled", particles_per_thread: int = 2, kernel_timing_every: int = 0,
+                 gather_particles_per_thread: int = 1, adjoint_p2g: str = "fused", scatter_chunks: str = "rows"):
         self.lib = _lib.load()
         scenes = list(scenes)
         if not scenes:
@@ -70,17 +76,54 @@ class Engine:
         if kernels not in self.KERNELS:
             raise ValueError(f"kernels must be one of {list(self.KERNELS)}")
         self.kernels = kernels
-        self._check(self.lib.ds_set_option(self.ctx, 1, self.KERNELS[kernels]))
-        self._check(self.lib.ds_set_option(self.ctx, 2, int(particles_per_thread)))
-        self._check(self.lib.ds_set_option(self.ctx, 3, int(kernel_timing_every)))
+        self._check(self.lib.ds_set_option(self.ctx, self.OPT_KERNELS, self.KERNELS[kernels]))
+        self._check(self.lib.ds_set_option(self.ctx, self.OPT_TIMING, int(kernel_timing_every)))
+        self.configure(particles_per_thread=particles_per_thread,
+                       gather_particles_per_thread=gather_particles_per_thread, adjoint_p2g=adjoint_p2g,
+                       scatter_chunks=scatter_chunks)
         for b, s in enumerate(scenes):
-            xl = _f32(s.x_liquid, (-1, 3))
-            xs = _f32(s.x_stick, (-1, 3))
-            xc = _f32(s.x_container, (-1, 3))
-            self._check(self.lib.ds_set_particles(self.ctx, b, _p32(xl), _p32(xs), _p32(xc)))
-            tr = _f32(s.traj[:self.T], (self.T, 3))
-            self._check(self.lib.ds_set_trajectory(self.ctx, b, _p32(tr)))
+            self.upload_scene(b, s)
         self.set_params(np.stack([np.asarray(s.params, dtype=np.float64) for s in scenes]))
+
+    def upload_scene(self, b: int, s: Scene):
+        """Initial particle positions and stick trajectory of instance b (same particle counts as
+        the scene the engine was built with); a trajectory shorter than the engine's T frames is
+        held at its last frame.  Takes effect at the next reset."""
+        xl = _f32(s.x_liquid, (-1, 3))
+        xs = _f32(s.x_stick, (-1, 3))
+        xc = _f32(s.x_container, (-1, 3))
+        self._check(self.lib.ds_set_particles(self.ctx, b, _p32(xl), _p32(xs), _p32(xc)))
+        tr = np.asarray(s.traj[:self.T], dtype=np.float32)
+        if tr.shape[0] < self.T:
+            tr = np.concatenate([tr, np.repeat(tr[-1:], self.T - tr.shape[0], axis=0)])
+        tr = _f32(tr, (self.T, 3))
+        self._check(self.lib.ds_set_trajectory(self.ctx, b, _p32(tr)))
+
+    def configure(self, particles_per_thread: Optional[int] = None, gather_particles_per_thread: Optional[int] = None,
+                  adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None):
+        """Execution options of the tiled kernels (they change how the substep runs, never what it
+        computes); particle-per-thread changes take effect at the next reset."""
+        if particles_per_thread is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_PPT, int(particles_per_thread)))
+            self.particles_per_thread = int(particles_per_thread)
+        if gather_particles_per_thread is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_GATHER_PPT, int(gather_particles_per_thread)))
+            self.gather_particles_per_thread = int(gather_particles_per_thread)
+        if adjoint_p2g is not None:
+            if adjoint_p2g not in self.ADJ_P2G:
+                raise ValueError(f"adjoint_p2g must be one of {list(self.ADJ_P2G)}")
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_ADJ_P2G, self.ADJ_P2G[adjoint_p2g]))
+            self.adjoint_p2g = adjoint_p2g
+        if scatter_chunks is not None:
+            if scatter_chunks not in self.SCATTER:
+                raise ValueError(f"scatter_chunks must be one of {list(self.SCATTER)}")
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_SCATTER, self.SCATTER[scatter_chunks]))
+            self.scatter_chunks = scatter_chunks
+
+    def options(self) -> dict:
+        return {"particles_per_thread": self.particles_per_thread,
+                "gather_particles_per_thread": self.gather_particles_per_thread, "adjoint_p2g": self.adjoint_p2g,
+                "scatter_chunks": self.scatter_chunks}
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):

@@ -46,7 +46,14 @@ enum {
     DS_OPT_KERNELS = 1,   /* DS_KERNELS_TILED (default) or DS_KERNELS_PER_PARTICLE; before ds_reset */
     DS_OPT_PARTICLES_PER_THREAD = 2,   /* tiled kernels: 1, 2 (default) or 4 particles per thread */
     DS_OPT_KERNEL_TIMING = 3,          /* time every N-th substep's kernels with HIP events (0 = off) */
+    DS_OPT_GATHER_PARTICLES_PER_THREAD = 4,  /* tiled gather kernels (G2P, adjoint P2G): 1 (default), 2, 4 */
+    DS_OPT_ADJOINT_P2G = 5,            /* DS_ADJ_P2G_FUSED (default) or DS_ADJ_P2G_SPLIT */
+    DS_OPT_SCATTER_CHUNKS = 6,         /* DS_SCATTER_ROWS (default) or DS_SCATTER_PACKED */
 };
+enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
+/* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or
+ * the packed chunks of the gather kernels, where lanes sharing a base take successive passes. */
+enum { DS_SCATTER_ROWS = 0, DS_SCATTER_PACKED = 1 };
 
 /* Kernels timed by DS_OPT_KERNEL_TIMING (tiled family). */
 enum {
@@ -55,9 +62,11 @@ enum {
     DS_K_G2P = 2,          /* k_g2p_tiled */
     DS_K_G2P_ADJ = 3,      /* k_g2p_adj_tiled */
     DS_K_VBAR_REDUCE = 4,  /* k_vbar_reduce */
-    DS_K_P2G_ADJ = 5,      /* k_p2g_adj_tiled */
+    DS_K_P2G_ADJ = 5,      /* k_p2g_adj_tiled: adjoint P2G gathers */
     DS_K_SORT = 6,         /* the whole bucket sort of one checkpoint */
-    DS_NKERNELS = 7
+    DS_K_P2G_ADJ_STRESS = 7, /* k_p2g_adj_stress: adjoint of the stress and of the FP64 SVD */
+    DS_K_G2P_FACTOR = 8,   /* k_g2p_tiled of the recompute, with the full FP64 SVD */
+    DS_NKERNELS = 9
 };
 enum { DS_KERNELS_TILED = 0, DS_KERNELS_PER_PARTICLE = 1 };
 

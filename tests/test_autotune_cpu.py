@@ -1,0 +1,74 @@
+"""The autotuner's search logic without a GPU: candidate enumeration, the choice, the cache."""
+import json
+
+import numpy as np
+import pytest
+
+from diffstir_amd import autotune as A
+from diffstir_amd import scene as S
+
+
+class _FakeEngine:
+    def __init__(self):
+        self.configured = []
+
+    def configure(self, **kw):
+        self.configured.append(kw)
+
+    def close(self):
+        pass
+
+
+def test_candidates_cover_the_space_once():
+    c = A.candidates()
+    # rows: 3 rows-per-wavefront x 2 gather sizes x 2 adjoint forms; packed: 2 x 2
+    assert len(c) == 3 * 2 * 2 + 2 * 2
+    assert len({tuple(sorted(d.items())) for d in c}) == len(c)
+    assert A.DEFAULT in c
+    assert all(d["particles_per_thread"] == A.DEFAULT["particles_per_thread"]
+               for d in c if d["scatter_chunks"] == "packed")
+
+
+def test_pick_takes_the_fastest_and_skips_failed_runs():
+    c = A.candidates()
+    times = [5.0, float("nan"), 3.0, 3.0, float("inf")] + [9.0] * (len(c) - 5)
+    best = A.pick(zip(c, times))
+    assert best == c[2]
+    with pytest.raises(RuntimeError):
+        A.pick([(c[0], float("nan"))])
+
+
+def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
+    A.clear_cache()
+    scenes = [S.tiny_scene(16, 20, seed=0)]
+    want = {"particles_per_thread": 4, "gather_particles_per_thread": 2, "adjoint_p2g": "split",
+            "scatter_chunks": "rows"}
+    calls = []
+
+    def timer(engine, cand, substeps, repeats):
+        calls.append((dict(cand), substeps))
+        return 1.0 if cand == want else 2.0 + len(calls)
+
+    eng = _FakeEngine()
+    f = tmp_path / "tune.json"
+    res = A.tune(scenes, ckpt_every=5, engine=eng, timer=timer, cache_file=str(f), warm_steps=0)
+    assert res["options"] == want
+    assert eng.configured[-1] == want
+    assert len(calls) == len(A.candidates()) and all(s == 10 for _, s in calls)   # 2 checkpoint segments
+    # cached in memory and on disk by signature
+    again = A.tune(scenes, ckpt_every=5, engine=eng, timer=lambda *a: pytest.fail("re-timed"))
+    assert again["options"] == want
+    disk = json.loads(f.read_text())
+    assert disk[res["signature"]]["options"] == want
+    A.clear_cache()
+    from_disk = A.tune(scenes, ckpt_every=5, engine=eng, timer=lambda *a: pytest.fail("re-timed"),
+                       cache_file=str(f))
+    assert from_disk["options"] == want
+    A.clear_cache()
+
+
+def test_signature_separates_workloads():
+    a = A.signature([S.tiny_scene(16, 20, seed=0)], 5)
+    b = A.signature([S.tiny_scene(16, 20, seed=0)] * 2, 5)
+    c = A.signature([S.tiny_scene(16, 20, seed=0)], 10)
+    assert len({a, b, c}) == 3

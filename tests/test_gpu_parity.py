@@ -140,20 +140,36 @@ def test_stick_alone_gives_centripetal_force(kernels):
     np.testing.assert_allclose(mag, M * w * w * R, rtol=2e-2)
 
 
-@pytest.mark.parametrize("ppt", [1, 2, 4])
-def test_particles_per_thread_variants_agree(ppt):
-    """The tiled kernels with 1, 2 or 4 particles per thread (chunks of 256, 512, 1024) give the
-    oracle's forces and gradient."""
+def test_every_execution_option_gives_the_oracle_result():
+    """Every candidate of the autotuner's search space (rows per wavefront 1/2/4 of the scatter
+    kernels, 1/2 particles per thread of the gather kernels, fused or split adjoint P2G) gives the
+    oracle's forces and gradient, on one context reconfigured between runs."""
+    from diffstir_amd.autotune import candidates
     sc, z = gold_scene("plain_n32_20")
-    print(f"[gpu] tiled, {ppt} particles per thread", flush=True)
-    with Engine([sc], ckpt_every=7, max_steps=21, particles_per_thread=ppt) as e:
-        e.reset()
-        e.forward()
-        F = e.forces()[0]
-        _, g = compute_loss(F, z["target"], sc.cfg.loss_kind)
-        d = e.backward(g[None])[0]
-    assert force_err(F, z["forces"]) <= FORCE_TOL
-    assert grad_err(d, z["dl_dparams"]) <= GRAD_TOL
+    with Engine([sc], ckpt_every=7, max_steps=21) as e:
+        for cand in candidates():
+            print(f"[gpu] tiled, options {cand}", flush=True)
+            e.configure(**cand)
+            e.reset()
+            e.forward()
+            F = e.forces()[0]
+            _, g = compute_loss(F, z["target"], sc.cfg.loss_kind)
+            d = e.backward(g[None])[0]
+            assert force_err(F, z["forces"]) <= FORCE_TOL, cand
+            assert grad_err(d, z["dl_dparams"]) <= GRAD_TOL, cand
+
+
+def test_autotune_picks_a_candidate_on_the_device():
+    from diffstir_amd import autotune as A
+    A.clear_cache()
+    sc = S.tiny_scene(16, 30, seed=4)
+    print("[gpu] autotune", flush=True)
+    res = A.tune([sc], ckpt_every=5, substeps=10, repeats=1)
+    assert res["options"] in A.candidates()
+    assert res["warm_steps"] == (sc.max_steps - 1) // 2
+    assert len(res["timings_ms"]) == len(A.candidates())
+    assert all(np.isfinite(t["ms"]) and t["ms"] > 0 for t in res["timings_ms"])
+    A.clear_cache()
 
 
 def test_tiled_matches_per_particle_kernels_over_a_longer_rollout():
