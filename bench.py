@@ -82,9 +82,18 @@ def grad_eval(eng: Engine, targets, kind):
     return losses, d
 
 
+def device_free_bytes() -> int:
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    hip.hipMemGetInfo(C.byref(free), C.byref(total))
+    return int(free.value)
+
+
 def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10,
             autotune: bool = True):
     scenes = S.workload(workload)
+    free0 = device_free_bytes()
     n_liq = sum(s.n_liquid for s in scenes)
     n_all = sum(s.n_liquid + s.n_stick + s.n_container for s in scenes)
     T = scenes[0].max_steps
@@ -115,14 +124,37 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
         log(f"[bench] step {i + 1}/{steps}: fwd {f:.1f} ms, adjoint {b:.1f} ms, loss[0] {losses[0]:.6g}")
     eng.synchronize()
     elapsed = time.perf_counter() - t0
+    mem_used = free0 - device_free_bytes()
     if not np.all(np.isfinite(d)):
         raise RuntimeError("non-finite gradient")
     psteps = float(n_liq) * (T - 1) * steps
     res = dict(workload=workload, scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps,
                fwd_ms=fwd_ms / steps, bwd_ms=bwd_ms / steps, B=len(scenes), kernels=eng.kernel_timing(),
-               options=eng.options() if kernels == "tiled" else None, tuned=tuned)
+               options=eng.options() if kernels == "tiled" else None, tuned=tuned, mem_used=mem_used)
     eng.close()
     return res
+
+
+def plain_single(workload: str, sample_steps: int, ckpt: int) -> dict:
+    """The plain one-instance GPU form: one thread per particle with global float atomics into a
+    dense grid (the per-particle kernel family), timed on the first `sample_steps` substeps of
+    the workload's first instance and scaled to its whole rollout."""
+    sc = S.workload(workload)[0]
+    T = sc.max_steps
+    short = sc.truncated(sample_steps + 1)
+    with Engine([short], ckpt_every=ckpt, kernels="per_particle") as e:
+        for _ in range(2):   # the first run warms up
+            e.reset()
+            t0 = time.perf_counter()
+            e.forward()
+            F = e.forces()
+            e.backward(np.ones_like(F))
+            e.synchronize()
+            el = time.perf_counter() - t0
+    per_eval = el * (T - 1) / sample_steps
+    return {"kernels": "per_particle", "workload": workload, "sample_substeps": sample_steps,
+            "gradient_evals_per_s": 1.0 / per_eval,
+            "fwd_adjoint_particle_steps_per_s": sc.n_liquid * sample_steps / el}
 
 
 def cpu_baseline(workload: str, sample_steps: int, ckpt: int):
@@ -231,7 +263,7 @@ def main():
                    "substeps": r["T"] - 1, "n_grid": r["scenes"][0].cfg.n_grid, "ckpt_every": a.ckpt,
                    "kernels": a.kernels, "options": r["options"], "autotuned": r["tuned"] is not None},
         "extra": {"gradient_evals_per_s": r["B"] * a.steps / r["elapsed"], "fwd_ms": r["fwd_ms"],
-                  "adjoint_ms": r["bwd_ms"],
+                  "adjoint_ms": r["bwd_ms"], "device_memory_GB": r["mem_used"] / 1e9,
                   "whole_evaluation_GBps": achieved,
                   "whole_evaluation_basis": f"{bpps:.1f} algorithmic B per fwd+adjoint particle-step"},
         "roofline": roofline(r),
@@ -245,6 +277,10 @@ def main():
             "fwd_adjoint_particle_steps_per_s": rb["psteps"] / rb["elapsed"],
             "ratio_to_single_instance_evals": (rb["B"] / rb["elapsed"]) / line["extra"]["gradient_evals_per_s"],
         }
+        plain = plain_single(a.workload, 200, a.ckpt)
+        line["extra"]["plain_single_instance"] = plain
+        line["extra"]["batched"]["ratio_to_plain_single_instance_evals"] = (
+            line["extra"]["batched"]["gradient_evals_per_s"] / plain["gradient_evals_per_s"])
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_sample_steps, a.ckpt)
     print(json.dumps(line), flush=True)

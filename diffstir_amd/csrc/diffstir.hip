@@ -44,6 +44,9 @@ struct ds_ctx {
     int gppt_sorted = 1;
     bool adj_split = false;            // adjoint P2G as two kernels (gather, stress) instead of one
     bool packed = false;               // scatter kernels on the packed (gather) chunks with claim passes
+    bool use_graph = false;            // replay whole-rollout forward / adjoint launch sequences as hipGraphs
+    bool capturing = false;            // a launch sequence is being captured (no kernel timing)
+    hipGraphExec_t fwd_exec = nullptr, bwd_exec = nullptr;
     int alloc_mode = -1;               // mode whose buffers are allocated
     std::vector<int> nl, ns, nc;
     std::vector<Group> groups;
@@ -97,6 +100,7 @@ struct ds_ctx {
     int* d_key = nullptr;
     int* d_rank = nullptr;
     int* d_rrpos = nullptr;
+    int* d_clist = nullptr;        // P: particles in cell-major order within buckets (sort scratch)
     Chunk* d_chunks = nullptr;    // ncp x maxch
     int* d_rows = nullptr;        // ncp x (P + 1) row table: first slot of every row
     GChunk* d_gchunks = nullptr;  // ncp x maxgch chunks of the gather-only kernels
@@ -113,8 +117,8 @@ struct ds_ctx {
     int* d_src = nullptr;         // ncp x P
     int NB = 0, NBK = 0;
     long maxch = 0, maxgch = 0;
-    std::vector<int> ngch_h;      // gather chunks per checkpoint
-    std::vector<int> nch_h;       // chunks per checkpoint
+    int* d_counts = nullptr;      // ncp x 4: sort totals of each checkpoint (particles, chunks, rows,
+                                  // gather chunks), read by the kernels; the host never reads them back
 
     // sampled per-kernel timing
     int kt_every = 0;
@@ -236,6 +240,8 @@ int check_flag(ds_ctx* c, const char* where)
     if (f) {
         unsigned z = 0;
         HIPCHK(c, hipMemcpy(c->d_flag, &z, sizeof(unsigned), hipMemcpyHostToDevice));
+        if (f & 2u)
+            return fail(c, DS_EHIP, std::string(where) + ": the bucket sort's totals do not match its tables");
         return fail(c, DS_ENONFINITE, std::string(where) +
                                           ": a particle left the grid's interior or became non-finite "
                                           "(the simulation is unstable for these parameters)");
@@ -292,8 +298,17 @@ void kt_collect(ds_ctx* c)
     c->kt_pending.clear();
 }
 
+// Drop captured launch sequences (after an option or buffer change).
+void drop_graphs(ds_ctx* c)
+{
+    if (c->fwd_exec) (void)hipGraphExecDestroy(c->fwd_exec);
+    if (c->bwd_exec) (void)hipGraphExecDestroy(c->bwd_exec);
+    c->fwd_exec = c->bwd_exec = nullptr;
+}
+
 void free_mode(ds_ctx* c)
 {
+    drop_graphs(c);
     dfree(c->d_seg);
     dfree(c->d_grid);
     dfree(c->d_vout);
@@ -321,8 +336,10 @@ void free_mode(ds_ctx* c)
     dfree(c->d_key);
     dfree(c->d_rank);
     dfree(c->d_rrpos);
+    dfree(c->d_clist);
     dfree(c->d_chunks);
     dfree(c->d_rows);
+    dfree(c->d_counts);
     dfree(c->d_gchunks);
     dfree(c->d_bgch);
     dfree(c->d_gstart);
@@ -395,7 +412,7 @@ int alloc_mode(ds_ctx* c)
         c->maxch = (long)c->P / WAVES + std::min<long>(c->P, c->NBK) + 1;
         c->maxgch = (long)c->P / WG + std::min<long>(c->P, c->NBK) + 1;
         // staging entries are addressed with 32-bit indices (slot * TNODES + node)
-        if ((c->maxch + c->maxch / 4 + 16) * (long)TNODES > 2147483647L)
+        if (std::max(c->maxch, c->maxgch) * (long)TNODES > 2147483647L)
             return fail(c, DS_EINVAL, "too many particles for the tiled kernels in one context");
         MALLOC(c->d_vgrid, G);
         MALLOC(c->d_pbar2, G);
@@ -429,10 +446,14 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_key, c->P);
         MALLOC(c->d_rank, c->P);
         MALLOC(c->d_rrpos, c->P);
+        MALLOC(c->d_clist, c->P);
         MALLOC(c->d_chunks, (size_t)c->maxch * c->ncp);
         MALLOC(c->d_src, (size_t)c->P * c->ncp);
-        c->nch_h.assign(c->ncp, 0);
-        c->ngch_h.assign(c->ncp, 0);
+        MALLOC(c->d_counts, (size_t)4 * c->ncp);
+        fill_i32(c, c->d_counts, 4L * c->ncp, 0);
+        // staging slots for either chunk set at its largest
+        c->staging_cap = std::max(c->maxch, c->maxgch);
+        MALLOC(c->d_staging, (size_t)c->staging_cap * TNODES);
         fill_i32(c, c->d_fbflag, (long)c->B * c->NB * c->NB * c->NB, 0);
         fill_zero_f4(c, c->d_pfb, G);
         fill_zero_f4(c, c->d_vbfb, G);
@@ -533,7 +554,13 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     hipLaunchKernelGGL(k_bucket_totals, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
                        c->d_celloff, c->d_bcount, c->d_brows, bnch, bgch, c->NBK, rpc, gch);
     Scan4 q{{c->d_bcount, bnch, c->d_brows, bgch}, {c->d_bstart, cstart, c->d_rstart, gstart}};
-    hipLaunchKernelGGL(k_scan4, dim3(1), dim3(1024), 0, c->stream, q, c->NBK, c->d_tot);
+    int* tot = c->d_counts + 4 * cp;
+    hipLaunchKernelGGL(k_scan4, dim3(1), dim3(1024), 0, c->stream, q, c->NBK, tot);
+    // deterministic ranks within each cell (by original particle index)
+    hipLaunchKernelGGL(k_cell_list, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_key, c->d_rank, c->d_celloff,
+                       c->d_bstart, c->d_clist, c->P);
+    hipLaunchKernelGGL(k_rank_fix, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_key, c->d_cnt, c->d_celloff,
+                       c->d_bstart, c->d_clist, c->d_pid, c->d_rank, c->P);
     const long nbl = (long)c->B * c->NB * c->NB * c->NB;
     int* cov = c->d_cov + (size_t)cp * nbl;
     fill_i32(c, cov, nbl, 0);
@@ -544,26 +571,20 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
                        c->d_rank, c->d_celloff, c->d_bstart, c->d_rrpos, c->d_meta, c->d_meta2, c->d_pid, c->d_pid2,
                        src, c->P);
-    std::swap(c->d_meta, c->d_meta2);
-    std::swap(c->d_pid, c->d_pid2);
-    int tot[4] = {0, 0, 0, 0};
-    HIPCHK(c, hipMemcpyAsync(tot, c->d_tot, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (tot[0] != c->nreal || tot[1] < 0 || tot[1] > c->maxch || tot[2] > c->nreal || tot[3] > c->maxgch)
-        return fail(c, DS_EHIP, "bucket sort lost particles (" + std::to_string(tot[0]) + " of " +
-                                    std::to_string(c->nreal) + ")");
-    c->nch_h[cp] = tot[1];
-    c->ngch_h[cp] = tot[3];
-    const long need = std::max(tot[1], tot[3]);   // staging slots of either chunk set
-    if (need > c->staging_cap) {
-        const long cap = need + need / 4 + 16;
-        dfree(c->d_staging);
-        c->staging_cap = 0;
-        RCHK(dalloc(c, &c->d_staging, (size_t)cap * TNODES));
-        c->staging_cap = cap;
-    }
+    // the sorted particle tables back into place (fixed buffers: a captured launch sequence
+    // replays with the same pointers)
+    hipLaunchKernelGGL(k_copy_i32, dim3(std::min<long>(nblocks(c->P), 4096)), dim3(256), 0, c->stream, c->d_meta,
+                       c->d_meta2, (long)c->P);
+    hipLaunchKernelGGL(k_copy_i32, dim3(std::min<long>(nblocks(c->P), 4096)), dim3(256), 0, c->stream, c->d_pid,
+                       c->d_pid2, (long)c->P);
+    hipLaunchKernelGGL(k_check_sort, dim3(1), dim3(64), 0, c->stream, tot, c->nreal, c->maxch, c->maxgch, c->d_flag);
     return DS_OK;
 }
+
+// Workgroups of a chunk kernel launch: the chunk count lives on the device, so the grid is fixed
+// (workgroups beyond the count exit at once; a count above the grid is covered by the loop).
+constexpr long CHUNK_GRID = 4096;
+inline dim3 chunk_grid(long maxchunks) { return dim3((unsigned)std::max<long>(1, std::min(maxchunks, CHUNK_GRID))); }
 
 #define DS_PPT_DISPATCH(ppt, kern, ...)                                                          \
     do {                                                                                         \
@@ -583,20 +604,21 @@ int sort_into_checkpoint(ds_ctx* c, int cp)
 void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* pout, float* aout)
 {
     // the scatter's chunk set: rows of distinct stencil bases, or packed gather chunks
-    const int nch = c->packed ? c->ngch_h[cp] : c->nch_h[cp];
     const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
     const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
-    if (nch > 0) {
+    const int* counts = c->d_counts + 4 * cp;
+    {
         KernelTimer kt(c, DS_K_P2G);
         if (c->packed)
-            DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_tiled, GChunk, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
-                             c->d_gchunks + (size_t)cp * c->maxgch, nullptr, nch, c->d_params, c->d_staging, c->d_pfb,
-                             c->d_fbflag, c->NB, c->d_j, c->dsc, c->P, c->d_flag);
-        else
-            DS_PPT_DISPATCH2(c->ppt_sorted, k_p2g_tiled, Chunk, dim3(nch), dim3(WG), 0, c->stream, in, fout, aout,
-                             c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1), nch,
+            DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_tiled, GChunk, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, in,
+                             fout, aout, c->d_gchunks + (size_t)cp * c->maxgch, nullptr, counts + 3, (int)c->maxgch,
                              c->d_params, c->d_staging, c->d_pfb, c->d_fbflag, c->NB, c->d_j, c->dsc, c->P,
                              c->d_flag);
+        else
+            DS_PPT_DISPATCH2(c->ppt_sorted, k_p2g_tiled, Chunk, chunk_grid(c->maxch), dim3(WG), 0, c->stream, in,
+                             fout, aout, c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1),
+                             counts + 1, (int)c->maxch, c->d_params, c->d_staging, c->d_pfb, c->d_fbflag, c->NB,
+                             c->d_j, c->dsc, c->P, c->d_flag);
     }
     const int nb3 = c->NB * c->NB * c->NB;
     KernelTimer kt(c, DS_K_GRID_REDUCE);
@@ -607,32 +629,32 @@ void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* p
 
 void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool record, double* mj = nullptr)
 {
-    const int ngc = c->ngch_h[cp];
-    if (ngc == 0) return;
     KernelTimer kt(c, mj ? DS_K_G2P_FACTOR : DS_K_G2P);
     const GChunk* gch = c->d_gchunks + (size_t)cp * c->maxgch;
+    const int* ngc = c->d_counts + 4 * cp + 3;
     double* dv = record ? c->d_dvsum : nullptr;
     if (mj)
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, true, dim3(ngc), dim3(WG), 0, c->stream, in, out, gch, ngc,
-                         c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P, c->d_flag);
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, true, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, in, out,
+                         gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P,
+                         c->d_flag);
     else
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, false, dim3(ngc), dim3(WG), 0, c->stream, in, out, gch, ngc,
-                         c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P, c->d_flag);
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, false, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, in, out,
+                         gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P,
+                         c->d_flag);
 }
 
 // J of every liquid particle of the checkpoint state `st` into d_j, for the segment's first P2G
 // (and, given mj, J and its SVD backward factor into mj).
 void liquid_j_tiled(ds_ctx* c, const float* st, int cp, double* mj = nullptr)
 {
-    const int ngc = c->ngch_h[cp];
-    if (ngc == 0) return;
     const GChunk* gch = c->d_gchunks + (size_t)cp * c->maxgch;
+    const int* ngc = c->d_counts + 4 * cp + 3;
     if (mj)
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, true, dim3(ngc), dim3(WG), 0, c->stream, st, gch, ngc, c->d_j, mj,
-                         c->dsc, c->P);
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, true, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, st, gch,
+                         ngc, (int)c->maxgch, c->d_j, mj, c->dsc, c->P);
     else
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, false, dim3(ngc), dim3(WG), 0, c->stream, st, gch, ngc, c->d_j, mj,
-                         c->dsc, c->P);
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, false, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, st, gch,
+                         ngc, (int)c->maxgch, c->d_j, mj, c->dsc, c->P);
 }
 
 int forward_tiled(ds_ctx* c, int nsteps)
@@ -642,7 +664,7 @@ int forward_tiled(ds_ctx* c, int nsteps)
         const int s = c->step;
         const int cp = s / c->K;
         const float* in = c->d_state;
-        c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
+        c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
         if (s % c->K == 0) {
             {
                 KernelTimer kt(c, DS_K_SORT);
@@ -664,14 +686,15 @@ int backward_tiled(ds_ctx* c)
     const long SF = c->stateFloats();
     const long G = c->gridNodes() * c->B;
     const int nb3 = c->NB * c->NB * c->NB;
+    const dim3 rgrid = chunk_grid(c->maxch), ggrid = chunk_grid(c->maxgch);
+    const int maxch = (int)c->maxch, maxgch = (int)c->maxgch;
     for (int cp = (S - 1) / c->K; cp >= 0; cp--) {
         const int s0 = cp * c->K;
         const int s1 = std::min(s0 + c->K, S);
         const int L = s1 - s0;
-        const int nch = c->nch_h[cp];
+        const int* counts = c->d_counts + 4 * cp;   // particles, row chunks, rows, gather chunks
         const Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
         const int* rows = c->d_rows + (size_t)cp * (c->P + 1);
-        const int ngc = c->ngch_h[cp];
         const GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
         const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
         const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
@@ -681,28 +704,28 @@ int backward_tiled(ds_ctx* c)
         auto aff_of = [&](int k) -> float* { return c->d_seg_aff + (size_t)k * 9 * c->P; };
         auto mj_of = [&](int k) -> double* { return c->d_seg_mj + (size_t)k * 10 * c->P; };
         // recompute the segment, keeping each step's state and momentum/mass grid
-        fill_zero_f64(c, c->d_thpart, 4L * std::max(ngc, 1));
+        fill_zero_f64(c, c->d_thpart, 4L * c->maxgch);
         liquid_j_tiled(c, ck, cp, mj_of(0));
         for (int k = 0; k < L; k++) {
             const int s = s0 + k;
-            c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
+            c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
             p2g_reduce_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k), aff_of(k));
             if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, s, false, mj_of(k + 1));
         }
         // adjoint, last step first
         for (int k = L - 1; k >= 0; k--) {
             const int s = s0 + k;
-            c->kt_on = c->kt_every > 0 && s % c->kt_every == 0;
-            if (c->packed && ngc > 0) {
+            c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
+            {
                 KernelTimer kt(c, DS_K_G2P_ADJ);
-                DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_adj_tiled, GChunk, dim3(ngc), dim3(WG), 0, c->stream, st_of(k),
-                                 c->d_adj, gchunks, nullptr, ngc, c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb,
-                                 c->d_fbflag, c->NB, c->dsc, c->P);
-            } else if (!c->packed && nch > 0) {
-                KernelTimer kt(c, DS_K_G2P_ADJ);
-                DS_PPT_DISPATCH2(c->ppt_sorted, k_g2p_adj_tiled, Chunk, dim3(nch), dim3(WG), 0, c->stream, st_of(k),
-                                 c->d_adj, chunks, rows, nch, c->d_dvbar, c->T, s, c->d_staging, c->d_vbfb,
-                                 c->d_fbflag, c->NB, c->dsc, c->P);
+                if (c->packed)
+                    DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_adj_tiled, GChunk, ggrid, dim3(WG), 0, c->stream, st_of(k),
+                                     c->d_adj, gchunks, nullptr, counts + 3, maxgch, c->d_dvbar, c->T, s,
+                                     c->d_staging, c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
+                else
+                    DS_PPT_DISPATCH2(c->ppt_sorted, k_g2p_adj_tiled, Chunk, rgrid, dim3(WG), 0, c->stream, st_of(k),
+                                     c->d_adj, chunks, rows, counts + 1, maxch, c->d_dvbar, c->T, s, c->d_staging,
+                                     c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
             }
             {
                 KernelTimer kt(c, DS_K_VBAR_REDUCE);
@@ -710,34 +733,38 @@ int backward_tiled(ds_ctx* c)
                                    c->d_cov + (size_t)cp * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2,
                                    c->dsc, c->NB);
             }
-            if (ngc > 0 && c->adj_split) {
+            if (c->adj_split) {
                 {
                     KernelTimer kt(c, DS_K_P2G_ADJ);
-                    DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, false, dim3(ngc), dim3(WG), 0, c->stream,
-                                     st_of(k), aff_of(k), c->d_adj, gchunks, ngc, grid_of(k), c->d_pbar2, c->d_gadj,
-                                     c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
+                    DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, false, ggrid, dim3(WG), 0, c->stream, st_of(k),
+                                     aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2,
+                                     c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
                 }
                 KernelTimer kt(c, DS_K_P2G_ADJ_STRESS);
-                DS_PPT_DISPATCH(c->gppt_sorted, k_p2g_adj_stress, dim3(ngc), dim3(WG), 0, c->stream, st_of(k),
-                                c->d_adj, gchunks, ngc, c->d_params, c->d_gadj, mj_of(k), c->d_thpart, c->dsc, c->P);
-            } else if (ngc > 0) {
+                DS_PPT_DISPATCH(c->gppt_sorted, k_p2g_adj_stress, ggrid, dim3(WG), 0, c->stream, st_of(k), c->d_adj,
+                                gchunks, counts + 3, maxgch, c->d_params, c->d_gadj, mj_of(k), c->d_thpart, c->dsc,
+                                c->P);
+            } else {
                 KernelTimer kt(c, DS_K_P2G_ADJ);
-                DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, true, dim3(ngc), dim3(WG), 0, c->stream, st_of(k),
-                                 aff_of(k), c->d_adj, gchunks, ngc, grid_of(k), c->d_pbar2, c->d_gadj, c->d_params,
-                                 mj_of(k), c->d_thpart, c->dsc, c->P);
+                DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, true, ggrid, dim3(WG), 0, c->stream, st_of(k),
+                                 aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2, c->d_gadj,
+                                 c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
             }
         }
-        if (ngc > 0)
-            hipLaunchKernelGGL(k_thpart_reduce, dim3((ngc + 255) / 256), dim3(256), 0, c->stream, gchunks, ngc,
-                               c->d_thpart, c->d_thbar);
+        hipLaunchKernelGGL(k_thpart_reduce, dim3((unsigned)std::min<long>((c->maxgch + 255) / 256, 1024)), dim3(256),
+                           0, c->stream, gchunks, counts + 3, maxgch, c->d_thpart, c->d_thbar);
         if (cp > 0) {
+            // the adjoint in the previous checkpoint's particle order, back into the fixed buffer
             hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
                                c->d_src + (size_t)cp * c->P, c->nreal, c->P);
-            std::swap(c->d_adj, c->d_adj2);
+            hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)std::min<long>(nblocks(SF), 8192)), dim3(256), 0, c->stream,
+                               c->d_adj, c->d_adj2, SF);
         }
     }
     return DS_OK;
 }
+
+int backward_tiled_n(ds_ctx* c, int) { return backward_tiled(c); }
 
 // Copy one species of one instance out of the current state, in the order it was given.
 int read_species(ds_ctx* c, int inst, int sp, int slot0, int ncomp, float* out)
@@ -897,19 +924,30 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         if (value != 1 && value != 2 && value != 4)
             return fail(c, DS_EINVAL, "particles per thread must be 1, 2 or 4");
         int& dst = option == DS_OPT_PARTICLES_PER_THREAD ? c->ppt : c->gppt;
-        if (value != dst) c->reset_done = false;
+        if (value != dst) {
+            c->reset_done = false;
+            drop_graphs(c);
+        }
         dst = value;
+        return DS_OK;
+    }
+    if (option == DS_OPT_GRAPH) {
+        if (value != 0 && value != 1) return fail(c, DS_EINVAL, "DS_OPT_GRAPH takes 0 or 1");
+        c->use_graph = value == 1;
+        if (!c->use_graph) drop_graphs(c);
         return DS_OK;
     }
     if (option == DS_OPT_SCATTER_CHUNKS) {
         if (value != DS_SCATTER_ROWS && value != DS_SCATTER_PACKED)
             return fail(c, DS_EINVAL, "DS_OPT_SCATTER_CHUNKS takes DS_SCATTER_ROWS or DS_SCATTER_PACKED");
+        if (c->packed != (value == DS_SCATTER_PACKED)) drop_graphs(c);
         c->packed = value == DS_SCATTER_PACKED;
         return DS_OK;
     }
     if (option == DS_OPT_ADJOINT_P2G) {
         if (value != DS_ADJ_P2G_FUSED && value != DS_ADJ_P2G_SPLIT)
             return fail(c, DS_EINVAL, "DS_OPT_ADJOINT_P2G takes DS_ADJ_P2G_FUSED or DS_ADJ_P2G_SPLIT");
+        if (c->adj_split != (value == DS_ADJ_P2G_SPLIT)) drop_graphs(c);
         c->adj_split = value == DS_ADJ_P2G_SPLIT;
         return DS_OK;
     }
@@ -979,15 +1017,50 @@ int ds_reset(ds_ctx* c)
     return DS_OK;
 }
 
+// Capture the launch sequence of body() on the context's stream into an executable graph
+// (nothing runs while capturing; the host-side step counter is restored).
+int capture_graph(ds_ctx* c, int (*body)(ds_ctx*, int), int arg, hipGraphExec_t* out)
+{
+    const int step0 = c->step;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    const int rc = body(c, arg);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->step = step0;
+    if (rc != DS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || !g) return fail(c, DS_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        *out = nullptr;
+        return fail(c, DS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    return DS_OK;
+}
+
 int ds_forward(ds_ctx* c, int32_t nsteps)
 {
     if (!c) return DS_EINVAL;
     if (!c->reset_done) return fail(c, DS_ESTATE, "ds_reset must be called before ds_forward");
     if (nsteps < 0 || c->step + nsteps > c->T - 1)
         return fail(c, DS_EINVAL, "nsteps exceeds the trajectory (max_steps - 1 substeps in total)");
+    const bool graph = c->use_graph && c->mode == DS_KERNELS_TILED && c->step == 0 && nsteps == c->T - 1;
+    if (graph && !c->fwd_exec) RCHK(capture_graph(c, forward_tiled, nsteps, &c->fwd_exec));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (c->mode == DS_KERNELS_TILED) RCHK(forward_tiled(c, nsteps));
-    else RCHK(forward_ref(c, nsteps));
+    if (graph) {
+        // the whole rollout (sorts included) as one graph launch
+        HIPCHK(c, hipGraphLaunch(c->fwd_exec, c->stream));
+        c->step += nsteps;
+    } else if (c->mode == DS_KERNELS_TILED) {
+        RCHK(forward_tiled(c, nsteps));
+    } else {
+        RCHK(forward_ref(c, nsteps));
+    }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -1028,10 +1101,13 @@ int ds_backward(ds_ctx* c, const double* dl_dforces, double* dl_dparams)
     const double scale = c->desc.m_stick / c->desc.dt;
     for (size_t i = 0; i < dvb.size(); i++) dvb[i] = scale * dl_dforces[i];
     HIPCHK(c, hipMemcpyAsync(c->d_dvbar, dvb.data(), sizeof(double) * dvb.size(), hipMemcpyHostToDevice, c->stream));
+    const bool graph = c->use_graph && c->mode == DS_KERNELS_TILED && c->step == c->T - 1;
+    if (graph && !c->bwd_exec) RCHK(capture_graph(c, backward_tiled_n, 0, &c->bwd_exec));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     fill_zero_f32(c, c->d_adj, c->stateFloats());
     fill_zero_f64(c, c->d_thbar, (long)c->B * 4);
-    if (c->mode == DS_KERNELS_TILED) RCHK(backward_tiled(c));
+    if (graph) HIPCHK(c, hipGraphLaunch(c->bwd_exec, c->stream));
+    else if (c->mode == DS_KERNELS_TILED) RCHK(backward_tiled(c));
     else RCHK(backward_ref(c));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));

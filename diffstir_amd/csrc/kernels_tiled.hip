@@ -229,12 +229,14 @@ template <int PPT, class CH>
 __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
                                                   float* __restrict__ aout, const CH* __restrict__ chunks,
                                                   const int* __restrict__ rows,
-                                                  int nchunks,
+                                                  const int* __restrict__ nchunks_p, int maxchunks,
                                                   const double* __restrict__ params, float4* __restrict__ staging,
                                                   float4* __restrict__ fallback, int* __restrict__ fbflag, int NB,
                                                   const double* __restrict__ jin, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
 {
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
     __shared__ __align__(16) float wt[WAVES * 4 * TILE];
     __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
@@ -412,12 +414,14 @@ __global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ s
 // factor M for the adjoint P2G of the next step in mj ([10][P]: M row-major, then J).
 template <int PPT, bool kFactor>
 __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
-                                                  const GChunk* __restrict__ chunks, int nchunks,
+                                                  const GChunk* __restrict__ chunks, const int* __restrict__ nchunks_p, int maxchunks,
                                                   const float4* __restrict__ vgrid, const float* __restrict__ traj,
                                                   int T, int f, double* __restrict__ dvsum, double* __restrict__ jout,
                                                   double* __restrict__ mj, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
 {
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
     __shared__ float tv[3 * TILE];
     __shared__ double red[WAVES][3];
     const int n = sc.n_grid;
@@ -527,9 +531,11 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
 // full SVD, also storing J and M in mj as k_g2p_tiled does.
 template <int PPT, bool kFactor>
 __global__ void __launch_bounds__(WG) k_liquid_j(const float* __restrict__ st, const GChunk* __restrict__ chunks,
-                                                 int nchunks, double* __restrict__ jout, double* __restrict__ mj,
+                                                 const int* __restrict__ nchunks_p, int maxchunks, double* __restrict__ jout, double* __restrict__ mj,
                                                  DevScene sc, int P)
 {
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const GChunk ch = chunks[c];
@@ -585,10 +591,12 @@ __device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __re
 template <int PPT, class CH>
 __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ st, const float* __restrict__ adj,
                                                       const CH* __restrict__ chunks, const int* __restrict__ rows,
-                                                      int nchunks, const double* __restrict__ dvbar, int T, int f,
+                                                      const int* __restrict__ nchunks_p, int maxchunks, const double* __restrict__ dvbar, int T, int f,
                                                       float4* __restrict__ staging, float4* __restrict__ fallback,
                                                       int* __restrict__ fbflag, int NB, DevScene sc, int P)
 {
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
     __shared__ __align__(16) float wt[WAVES * 3 * TILE];
     __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
@@ -761,11 +769,13 @@ __device__ __forceinline__ void add_chunk_partials(double (*red)[4], const doubl
 template <int PPT, bool kFused>
 __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, const float* __restrict__ ain,
                                                       float* __restrict__ adj, const GChunk* __restrict__ chunks,
-                                                      int nchunks, const float4* __restrict__ grid,
+                                                      const int* __restrict__ nchunks_p, int maxchunks, const float4* __restrict__ grid,
                                                       const float4* __restrict__ pbar, float* __restrict__ gout,
                                                       const double* __restrict__ params, const double* __restrict__ mj,
                                                       double* __restrict__ thpart, DevScene sc, int P)
 {
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
     __shared__ float tq[4 * TILE];
     __shared__ float tv[3 * TILE];
     __shared__ double red[WAVES][4];
@@ -878,11 +888,13 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
 // workgroup sums into the chunk's slot of thpart).
 template <int PPT>
 __global__ void __launch_bounds__(WG) k_p2g_adj_stress(const float* __restrict__ st, float* __restrict__ adj,
-                                                       const GChunk* __restrict__ chunks, int nchunks,
+                                                       const GChunk* __restrict__ chunks, const int* __restrict__ nchunks_p, int maxchunks,
                                                        const double* __restrict__ params,
                                                        const float* __restrict__ gin, const double* __restrict__ mj,
                                                        double* __restrict__ thpart, DevScene sc, int P)
 {
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
     __shared__ double red[WAVES][4];
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -914,38 +926,42 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_stress(const float* __restrict__
 // The per-chunk parameter-gradient partials of a segment into the per-instance gradient: a
 // workgroup sums the partials of its chunks with one atomic per parameter when they all belong
 // to one instance (the usual case: chunks are ordered by instance), else one per chunk.
-__global__ void __launch_bounds__(256) k_thpart_reduce(const GChunk* __restrict__ chunks, int nchunks,
+__global__ void __launch_bounds__(256) k_thpart_reduce(const GChunk* __restrict__ chunks,
+                                                       const int* __restrict__ nchunks_p, int maxchunks,
                                                        const double* __restrict__ thpart, double* __restrict__ thbar)
 {
     __shared__ double red[WAVES][4];
     __shared__ int inst0;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int last = min(nchunks, (int)(blockIdx.x + 1) * (int)blockDim.x) - 1;
-    if (threadIdx.x == 0) inst0 = chunks[blockIdx.x * blockDim.x].inst;
-    __syncthreads();
-    const bool ok = c < nchunks;
-    const int inst = ok ? chunks[c].inst : inst0;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    if (ok) {
+    const int nchunks = min(*nchunks_p, maxchunks);
+    for (int base = blockIdx.x * blockDim.x; base < nchunks; base += gridDim.x * blockDim.x) {
+        const int c = base + threadIdx.x;
+        if (threadIdx.x == 0) inst0 = chunks[base].inst;
+        __syncthreads();
+        const bool ok = c < nchunks;
+        const int inst = ok ? chunks[c].inst : inst0;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        if (ok) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = thpart[4 * (long)c + k];
-    }
-    if (__syncthreads_and(inst == inst0) && chunks[last].inst == inst0) {
-        const int w = threadIdx.x >> 6;
+            for (int k = 0; k < 4; k++) v[k] = thpart[4 * (long)c + k];
+        }
+        if (__syncthreads_and(inst == inst0)) {
+            const int w = threadIdx.x >> 6;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const double r = wave_sum(v[k]);
-            if ((threadIdx.x & 63) == 0) red[w][k] = r;
+            for (int k = 0; k < 4; k++) {
+                const double r = wave_sum(v[k]);
+                if ((threadIdx.x & 63) == 0) red[w][k] = r;
+            }
+            __syncthreads();
+            if (threadIdx.x < 4) {
+                double a = 0.0;
+                for (int k = 0; k < WAVES; k++) a += red[k][threadIdx.x];
+                atomicAdd(thbar + 4 * inst0 + threadIdx.x, a);
+            }
+        } else if (ok) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) atomicAdd(thbar + 4 * inst + k, v[k]);
         }
         __syncthreads();
-        if (threadIdx.x < 4) {
-            double a = 0.0;
-            for (int k = 0; k < WAVES; k++) a += red[k][threadIdx.x];
-            atomicAdd(thbar + 4 * inst0 + threadIdx.x, a);
-        }
-    } else if (ok) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) atomicAdd(thbar + 4 * inst + k, v[k]);
     }
 }
 
@@ -1137,6 +1153,37 @@ __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ c
 }
 
 // Move every particle to its sorted slot; src[q] records where slot q came from.
+// Particle p into the cell-major list of its bucket at its provisional rank (the order in which
+// the counting atomics happened to run).
+__global__ void __launch_bounds__(256) k_cell_list(const int* __restrict__ key, const int* __restrict__ rank,
+                                                   const int* __restrict__ cell_off, const int* __restrict__ bstart,
+                                                   int* __restrict__ clist, int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int cell = key[p];
+    if (cell < 0) return;
+    clist[bstart[cell / 64] + cell_off[cell] + rank[p]] = p;
+}
+
+// The rank of particle p in its cell by original particle index, so that the sorted order (and
+// with it every summation order downstream) does not depend on the order of the atomics.
+__global__ void __launch_bounds__(256) k_rank_fix(const int* __restrict__ key, const int* __restrict__ cnt,
+                                                  const int* __restrict__ cell_off, const int* __restrict__ bstart,
+                                                  const int* __restrict__ clist, const int* __restrict__ pid,
+                                                  int* __restrict__ rank, int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int cell = key[p];
+    if (cell < 0) return;
+    const int* l = clist + bstart[cell / 64] + cell_off[cell];
+    const int n = cnt[cell], me = pid[p];
+    int r = 0;
+    for (int j = 0; j < n; j++) r += pid[l[j]] < me ? 1 : 0;
+    rank[p] = r;
+}
+
 __global__ void __launch_bounds__(256) k_sort_scatter(const float* __restrict__ in, float* __restrict__ out,
                                                       const int* __restrict__ key, const int* __restrict__ rank,
                                                       const int* __restrict__ cell_off, const int* __restrict__ bstart,
@@ -1156,6 +1203,29 @@ __global__ void __launch_bounds__(256) k_sort_scatter(const float* __restrict__ 
     meta_out[q] = meta_in[p];
     pid_out[q] = pid_in[p];
     src[q] = p;
+}
+
+// The bucket sort's totals against what the tables hold: every particle placed, chunk and row
+// counts within their tables; otherwise flag bit 2 (reported by the host after the call).
+__global__ void k_check_sort(const int* __restrict__ tot, int nreal, long maxch, long maxgch,
+                             unsigned* __restrict__ flag)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (tot[0] != nreal || tot[1] < 0 || tot[1] > maxch || tot[2] > nreal || tot[3] < 0 || tot[3] > maxgch)
+            atomicOr(flag, 2u);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_copy_i32(int* __restrict__ dst, const int* __restrict__ src, long n)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256) k_copy_f32(float* __restrict__ dst, const float* __restrict__ src, long n)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 __global__ void __launch_bounds__(256) k_fill_i32(int* __restrict__ p, long n, int v)

@@ -45,13 +45,14 @@ class Engine:
     KERNELS = {"tiled": 0, "per_particle": 1}
 
     # execution options of the tiled kernels (ds_set_option ids)
-    OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER = 1, 2, 3, 4, 5, 6
+    OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
     def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None,
                  kernels: str = "tiled", particles_per_thread: int = 2, kernel_timing_every: int = 0,
-                 gather_particles_per_thread: int = 1, adjoint_p2g: str = "fused", scatter_chunks: str = "rows"):
+                 gather_particles_per_thread: int = 1, adjoint_p2g: str = "fused", scatter_chunks: str = "rows",
+                 graph: bool = False):
         self.lib = _lib.load()
         scenes = list(scenes)
         if not scenes:
@@ -81,6 +82,7 @@ class Engine:
         self.configure(particles_per_thread=particles_per_thread,
                        gather_particles_per_thread=gather_particles_per_thread, adjoint_p2g=adjoint_p2g,
                        scatter_chunks=scatter_chunks)
+        self.set_graph(graph)
         for b, s in enumerate(scenes):
             self.upload_scene(b, s)
         self.set_params(np.stack([np.asarray(s.params, dtype=np.float64) for s in scenes]))
@@ -119,6 +121,12 @@ class Engine:
                 raise ValueError(f"scatter_chunks must be one of {list(self.SCATTER)}")
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_SCATTER, self.SCATTER[scatter_chunks]))
             self.scatter_chunks = scatter_chunks
+
+    def set_graph(self, on: bool):
+        """Replay a whole-rollout forward() and the backward() after it as hipGraphs captured at
+        their first call (per-kernel timing does not sample inside them)."""
+        self._check(self.lib.ds_set_option(self.ctx, self.OPT_GRAPH, 1 if on else 0))
+        self.graph = bool(on)
 
     def options(self) -> dict:
         return {"particles_per_thread": self.particles_per_thread,

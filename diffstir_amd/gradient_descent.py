@@ -105,19 +105,21 @@ def single_parameter_fitting(parameter: float, parameter_grad: float, lr: float,
 
 def parameters_fitting(sim: Simulator, epoch_setting: int, lower_bound: Sequence[float],
                        upper_bound: Sequence[float], verbose: bool = False,
-                       n_fit: Optional[int] = None) -> Tuple[List[float], dict]:
-    """gdm:84-139 for the first clip: 3 * epoch_setting taped epochs with the step rule above.
+                       n_fit: Optional[int] = None, first_clip: bool = True) -> Tuple[List[float], dict]:
+    """gdm:84-139 for one clip: taped epochs with the step rule above, 3 * epoch_setting of them
+    for the first clip (gdm:96-98), epoch_setting for later ones.
 
     Returns (best parameters, history with losses, parameters and gradients per epoch).
     `n_fit` limits the fitted parameters to the first n (Herschel-Bulkley leaves params[3]
     unused when the bulk modulus is fixed)."""
-    sim.initialize_objects()
+    if first_clip:
+        sim.initialize_objects()
     n_par = 4 if n_fit is None else n_fit
     lr = [1.0] * 4
     grad_list = [0.0] * 4
     grad_shock = [0] * 4
     losses, params_hist, grads_hist = [], [], []
-    epochs = epoch_setting * 3
+    epochs = epoch_setting * 3 if first_clip else epoch_setting
     lo = [float(v) for v in lower_bound]
     hi = [float(v) for v in upper_bound]
     for i in range(epochs):
@@ -143,3 +145,30 @@ def parameters_fitting(sim: Simulator, epoch_setting: int, lower_bound: Sequence
     sim.constitutive_parameters[:] = params_hist[best]
     sim.loss = losses[best]
     return list(params_hist[best]), {"losses": losses, "params": params_hist, "grads": grads_hist, "best": best}
+
+
+def fit_clips(sim: Simulator, trajectory: np.ndarray, forces: np.ndarray, n_clips: int, epoch_setting: int,
+              lower_bound: Sequence[float], upper_bound: Sequence[float], verbose: bool = False,
+              n_fit: Optional[int] = None) -> Tuple[List[float], List[dict]]:
+    """The clip loop of gdm:86-139 as it evidently intends: clip c fits against frames
+    [c F, (c + 1) F) of the trajectory and target (F = the simulator's max_steps), starting from
+    the particle positions the previous clip's last rollout ended with (gdm:131-138), and hands
+    its best parameters on.  The reference returns inside the loop after clip 0 (gdm:139) and
+    slices the same frames for the next clip as for the current one (gdm:134-135); neither is
+    reproduced here.  Returns (the last clip's best parameters, per-clip histories)."""
+    F = sim.max_steps
+    trajectory = np.asarray(trajectory)
+    forces = np.asarray(forces, dtype=np.float64)
+    if trajectory.shape[0] < n_clips * F or forces.shape[0] < n_clips * F:
+        raise ValueError(f"{n_clips} clips of {F} frames need {n_clips * F} trajectory and target frames")
+    hists = []
+    best = list(sim.constitutive_parameters)
+    for clip in range(n_clips):
+        if clip > 0:
+            sim.hand_off_clip(trajectory[clip * F:(clip + 1) * F], forces[clip * F:(clip + 1) * F])
+        else:
+            sim.operator_forces_seq = forces[:F]
+        best, hist = parameters_fitting(sim, epoch_setting, lower_bound, upper_bound, verbose=verbose, n_fit=n_fit,
+                                        first_clip=clip == 0)
+        hists.append(hist)
+    return best, hists

@@ -267,7 +267,8 @@ def stirring_scene(n_grid: int = 32, substeps: int = 1000, dt: float = 2e-4, fil
     """A cylinder of liquid (8 particles per cell) stirred by a particle-sampled vertical stick.
 
     The fill's bottom sits at y = bound*dx; the cylindrical slip wall is 2 cells outside the
-    fill; the stick spans from 2 cells above the floor to 4 cells above the fill's top."""
+    fill; the stick spans from 2 cells above the floor to 4 cells above the fill's top, raised by
+    the vertical oscillation's amplitude so that its lowest point stays 2 cells above the floor."""
     rng = np.random.default_rng(seed)
     dx = 1.0 / n_grid
     bound = 3
@@ -276,7 +277,9 @@ def stirring_scene(n_grid: int = 32, substeps: int = 1000, dt: float = 2e-4, fil
     frames = substeps + 1
     traj = stick_path(frames, dt, path_rx, path_rz, omega, y_amp, y_freq)
     s0 = traj[0]
-    stick = jittered_cylinder(rng, n_grid, s0[0], s0[2], stick_radius, y0 + 2 * dx, y0 + fill_height + 4 * dx)
+    lift = abs(y_amp)
+    stick = jittered_cylinder(rng, n_grid, s0[0], s0[2], stick_radius, y0 + 2 * dx + lift,
+                              y0 + fill_height + 4 * dx + lift)
     stick[:, 1] += s0[1]
     p_vol = dx ** 3 / 8.0
     if params is None:

@@ -72,6 +72,28 @@ class Simulator:
     def step(self) -> int:
         return self.engine.step
 
+    def hand_off_clip(self, trajectory: np.ndarray, target: Optional[np.ndarray] = None):
+        """The clip hand-off of gdm:131-138: the next clip starts from the particle positions of
+        the last rollout's final step (initial_x, initial_operator_x), with the next trajectory
+        frames and target; then initialize_ti_field.  The reference writes a clip-long slice
+        into max_steps-long fields there (a shape mismatch); here the slice must have the
+        simulator's max_steps frames."""
+        import dataclasses
+        trajectory = np.asarray(trajectory, dtype=np.float32)
+        if trajectory.shape != (self.max_steps, 3):
+            raise ValueError(f"the next clip's trajectory must have {self.max_steps} frames, got {trajectory.shape}")
+        xl = self.engine.read_liquid(0)["x"]
+        xs = self.engine.read_stick(0)["x"] if self.scene.n_stick else self.scene.x_stick
+        self.scene = dataclasses.replace(self.scene, x_liquid=xl, x_stick=xs, traj=trajectory)
+        self.engine.scenes[0] = self.scene
+        self.engine.upload_scene(0, self.scene)
+        if target is not None:
+            target = np.asarray(target, dtype=np.float64)
+            if target.shape[0] != self.max_steps:
+                raise ValueError(f"the next clip's target must have {self.max_steps} rows, got {target.shape}")
+            self.operator_forces_seq = target
+        self.initialize_ti_field()
+
     # ------------------------------------------------------------------ substep
     def forward(self, s: int):
         """One substep s -> s+1 (sf:341-348).  Steps must be taken in order from 0."""

@@ -49,6 +49,9 @@ enum {
     DS_OPT_GATHER_PARTICLES_PER_THREAD = 4,  /* tiled gather kernels (G2P, adjoint P2G): 1 (default), 2, 4 */
     DS_OPT_ADJOINT_P2G = 5,            /* DS_ADJ_P2G_FUSED (default) or DS_ADJ_P2G_SPLIT */
     DS_OPT_SCATTER_CHUNKS = 6,         /* DS_SCATTER_ROWS (default) or DS_SCATTER_PACKED */
+    DS_OPT_GRAPH = 7,                  /* 1: a whole-rollout ds_forward (from step 0 to the end) and the
+                                          ds_backward after it replay hipGraphs captured at their first
+                                          call (no per-kernel timing inside them); 0 (default): launches */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or

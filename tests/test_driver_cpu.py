@@ -91,3 +91,15 @@ def test_batch_workload_draws():
         K, ty, n, k = s.params
         assert 0.1 <= K <= 5 and 0 <= ty <= 20 and 0.5 <= n <= 1.5 and k == 1e3
         assert s.max_steps == 2001
+
+
+def test_oscillating_stick_stays_inside_the_grid():
+    """large_n128's stick moves up and down by 0.05: its lowest point over the whole trajectory
+    stays above the floor's bound cells (the kernels flag any particle leaving the interior)."""
+    sc = S.stirring_scene(32, 4000, fill_radius=0.35, fill_height=0.3, stick_radius=0.02, path_rx=0.2,
+                          path_rz=0.1, y_amp=0.05, y_freq=1.0, seed=0)
+    dx = sc.cfg.dx
+    lowest = sc.x_stick[:, 1].min() + (sc.traj[:, 1] - sc.traj[0, 1]).min()
+    assert lowest >= (sc.cfg.bound + 2) * dx - 1e-6
+    highest = sc.x_stick[:, 1].max() + (sc.traj[:, 1] - sc.traj[0, 1]).max()
+    assert highest <= 1.0 - sc.cfg.bound * dx

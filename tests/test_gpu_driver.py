@@ -121,3 +121,31 @@ def test_batched_identification_reduces_the_loss():
     L = res["losses"]
     assert L.shape == (12, 4)
     assert np.all(L[-1] < L[0])
+
+
+def test_clip_hand_off_continues_from_the_last_state():
+    """gdm:131-138 as intended: the second clip starts where the first clip's last rollout ended
+    (liquid and stick positions), with the next trajectory frames and target."""
+    from diffstir_amd.gradient_descent import fit_clips
+    sc = S.tiny_scene(16, 10, seed=3)
+    # a 2-clip trajectory: the first clip's frames, then the path continued (same circle)
+    full = S.tiny_scene(16, 19, seed=3).traj
+    traj = np.concatenate([full[:10], full[9:19]])
+    target = np.zeros((20, 3))
+    print("[gpu] clip hand-off", flush=True)
+    sim = Simulator(dataclasses.replace(sc, traj=traj[:10]), ckpt_every=5)
+    sim.initialize_objects()
+    sim.forward_steps()
+    end = sim.engine.read_liquid(0)["x"].copy()
+    end_stick = sim.engine.read_stick(0)["x"].copy()
+    sim.hand_off_clip(traj[10:20], target[10:20])
+    assert sim.step == 0
+    np.testing.assert_array_equal(sim.engine.read_liquid(0)["x"], end)
+    np.testing.assert_array_equal(sim.engine.read_stick(0)["x"], end_stick)
+    sim.forward_steps()
+    assert np.all(np.isfinite(sim.engine.forces()))
+    with pytest.raises(ValueError):
+        sim.hand_off_clip(traj[10:15])
+    best, hists = fit_clips(sim, traj, target, 2, 1, [0.1, 0.0, 0.5, 0.0], [5.0, 20.0, 1.5, 1e4], n_fit=3)
+    assert len(hists) == 2 and len(hists[0]["losses"]) == 3 and len(hists[1]["losses"]) == 1
+    sim.close()
