@@ -91,7 +91,7 @@ def device_free_bytes() -> int:
 
 
 def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10,
-            autotune: bool = True):
+            autotune: bool = True, graph: bool = True, kernel_sample: bool = True):
     scenes = S.workload(workload)
     free0 = device_free_bytes()
     n_liq = sum(s.n_liquid for s in scenes)
@@ -99,13 +99,15 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
     T = scenes[0].max_steps
     log(f"[bench] workload {workload}: {len(scenes)} instance(s), {n_liq} liquid particles, {T - 1} substeps, "
         f"checkpoint every {ckpt}")
-    eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels, kernel_timing_every=timing_every if kernels == "tiled" else 0)
+    eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels)
     tuned = None
     if autotune and kernels == "tiled":
         from diffstir_amd.autotune import tune
         t0 = time.perf_counter()
         tuned = tune(scenes, ckpt_every=ckpt, engine=eng)
         log(f"[bench] autotuned in {time.perf_counter() - t0:.1f} s: {tuned['options']}")
+    graph = graph and kernels == "tiled"
+    eng.set_graph(graph)
     targets = make_targets(eng, scenes)
     eng.set_params(candidate_params(scenes))
     kind = scenes[0].cfg.loss_kind
@@ -113,7 +115,6 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
         grad_eval(eng, targets, kind)
         log(f"[bench] warmup {i + 1}/{warmup} done")
     eng.synchronize()
-    eng.kernel_timing(reset=True)
     fwd_ms = bwd_ms = 0.0
     t0 = time.perf_counter()
     for i in range(steps):
@@ -127,10 +128,21 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
     mem_used = free0 - device_free_bytes()
     if not np.all(np.isfinite(d)):
         raise RuntimeError("non-finite gradient")
+    kt = {}
+    if kernel_sample and kernels == "tiled":
+        # one more evaluation, outside the timed region, launched kernel by kernel with every
+        # timing_every-th substep's kernels timed by HIP events (graph replays are not sampled)
+        eng.set_graph(False)
+        eng.set_kernel_timing(timing_every)
+        eng.kernel_timing(reset=True)
+        grad_eval(eng, targets, kind)
+        eng.synchronize()
+        kt = eng.kernel_timing()
+        eng.set_kernel_timing(0)
     psteps = float(n_liq) * (T - 1) * steps
     res = dict(workload=workload, scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps,
-               fwd_ms=fwd_ms / steps, bwd_ms=bwd_ms / steps, B=len(scenes), kernels=eng.kernel_timing(),
-               options=eng.options() if kernels == "tiled" else None, tuned=tuned, mem_used=mem_used)
+               fwd_ms=fwd_ms / steps, bwd_ms=bwd_ms / steps, B=len(scenes), kernels=kt,
+               options=eng.options() if kernels == "tiled" else None, tuned=tuned, mem_used=mem_used, graph=graph)
     eng.close()
     return res
 
@@ -233,6 +245,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     a = ap.parse_args()
     if a.calibrate_copy_mib:
         calibrate_copy(a.calibrate_copy_mib)
@@ -241,7 +254,7 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the instances of a batch share one device "
               "and nothing here uses a second GPU", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune)
+    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph)
     value = r["psteps"] / r["elapsed"]
     ms_per_step = 1000.0 * r["elapsed"] / a.steps
     bpps = fwd_adj_bytes(a.ckpt)
@@ -261,7 +274,8 @@ def main():
         "data": "synthetic (seeded cylinder fill, circular stick path)",
         "config": {"workload": a.workload, "instances": r["B"], "liquid_particles": r["n_liq"],
                    "substeps": r["T"] - 1, "n_grid": r["scenes"][0].cfg.n_grid, "ckpt_every": a.ckpt,
-                   "kernels": a.kernels, "options": r["options"], "autotuned": r["tuned"] is not None},
+                   "kernels": a.kernels, "options": r["options"], "autotuned": r["tuned"] is not None,
+                   "hip_graph": r["graph"]},
         "extra": {"gradient_evals_per_s": r["B"] * a.steps / r["elapsed"], "fwd_ms": r["fwd_ms"],
                   "adjoint_ms": r["bwd_ms"], "device_memory_GB": r["mem_used"] / 1e9,
                   "whole_evaluation_GBps": achieved,
@@ -269,7 +283,8 @@ def main():
         "roofline": roofline(r),
     }
     if not a.no_batch:
-        rb = run_gpu(a.batch_workload, 1, 1, a.ckpt, a.kernels, autotune=not a.no_autotune)
+        rb = run_gpu(a.batch_workload, 1, 1, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph,
+                     kernel_sample=False)
         line["extra"]["batched"] = {
             "workload": a.batch_workload, "instances": rb["B"], "liquid_particles": rb["n_liq"],
             "options": rb["options"],

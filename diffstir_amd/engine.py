@@ -122,6 +122,10 @@ class Engine:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_SCATTER, self.SCATTER[scatter_chunks]))
             self.scatter_chunks = scatter_chunks
 
+    def set_kernel_timing(self, every: int):
+        """Sample every N-th substep's kernels with HIP events (0: off)."""
+        self._check(self.lib.ds_set_option(self.ctx, self.OPT_TIMING, int(every)))
+
     def set_graph(self, on: bool):
         """Replay a whole-rollout forward() and the backward() after it as hipGraphs captured at
         their first call (per-kernel timing does not sample inside them)."""
