@@ -91,15 +91,19 @@ def device_free_bytes() -> int:
 
 
 def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10,
-            autotune: bool = True, graph: bool = True, kernel_sample: bool = True):
+            autotune: bool = True, graph: bool = True, kernel_sample: bool = True, substeps: int = 0,
+            overlap: bool = True):
     scenes = S.workload(workload)
+    if substeps:
+        # a shortened rollout (profiling runs whose every launch is serialised by the profiler)
+        scenes = [s.truncated(min(substeps + 1, s.max_steps)) for s in scenes]
     free0 = device_free_bytes()
     n_liq = sum(s.n_liquid for s in scenes)
     n_all = sum(s.n_liquid + s.n_stick + s.n_container for s in scenes)
     T = scenes[0].max_steps
     log(f"[bench] workload {workload}: {len(scenes)} instance(s), {n_liq} liquid particles, {T - 1} substeps, "
         f"checkpoint every {ckpt}")
-    eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels)
+    eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels, overlap=overlap)
     tuned = None
     if autotune and kernels == "tiled":
         from diffstir_amd.autotune import tune
@@ -246,6 +250,10 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run the adjoint's segment recompute on the main stream (no second stream)")
+    ap.add_argument("--substeps", type=int, default=0,
+                    help="shorten the rollout to this many substeps (profiling only; 0 = the workload's own)")
     a = ap.parse_args()
     if a.calibrate_copy_mib:
         calibrate_copy(a.calibrate_copy_mib)
@@ -254,7 +262,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the instances of a batch share one device "
               "and nothing here uses a second GPU", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph)
+    r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph,
+                substeps=a.substeps, overlap=not a.no_overlap)
     value = r["psteps"] / r["elapsed"]
     ms_per_step = 1000.0 * r["elapsed"] / a.steps
     bpps = fwd_adj_bytes(a.ckpt)
@@ -284,7 +293,7 @@ def main():
     }
     if not a.no_batch:
         rb = run_gpu(a.batch_workload, 1, 1, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph,
-                     kernel_sample=False)
+                     kernel_sample=False, overlap=not a.no_overlap)
         line["extra"]["batched"] = {
             "workload": a.batch_workload, "instances": rb["B"], "liquid_particles": rb["n_liq"],
             "options": rb["options"],

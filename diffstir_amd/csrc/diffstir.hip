@@ -17,6 +17,7 @@
 //                        seg_grid                     K x B x G float4 (momentum/mass per step)
 //                        bucket sort scratch, chunk tables and re-sort maps per checkpoint
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -45,6 +46,7 @@ struct ds_ctx {
     bool adj_split = false;            // adjoint P2G as two kernels (gather, stress) instead of one
     bool packed = false;               // scatter kernels on the packed (gather) chunks with claim passes
     bool use_graph = false;            // replay whole-rollout forward / adjoint launch sequences as hipGraphs
+    bool overlap = true;               // adjoint: recompute segment k-1 on a second stream during segment k's adjoint
     bool capturing = false;            // a launch sequence is being captured (no kernel timing)
     hipGraphExec_t fwd_exec = nullptr, bwd_exec = nullptr;
     int alloc_mode = -1;               // mode whose buffers are allocated
@@ -112,6 +114,16 @@ struct ds_ctx {
     double* d_seg_mj = nullptr;   // K x 10 x P: J and its SVD backward factor per step of a segment
     float* d_gadj = nullptr;      // 18 x P: gathered quantities of the adjoint P2G
     double* d_thpart = nullptr;   // maxgch x 4: per-chunk parameter-gradient partials of a segment
+    // second segment buffer set and the recompute's own scratch, for the overlapped adjoint
+    float* d_seg_b = nullptr;
+    float4* d_seg_grid_b = nullptr;
+    float* d_seg_aff_b = nullptr;
+    double* d_seg_mj_b = nullptr;
+    float4* d_staging_r = nullptr;
+    int* d_fbflag_r = nullptr;
+    bool overlap_alloc = false;
+    hipStream_t stream2 = nullptr;     // the recompute stream (default priority, like the main stream)
+    hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr};
     int* d_brows = nullptr;       // rows per bucket
     int* d_rstart = nullptr;      // first row of each bucket
     int* d_src = nullptr;         // ncp x P
@@ -263,25 +275,44 @@ hipEvent_t kt_event(ds_ctx* c)
     return e;
 }
 
-// Bracket a launch with events when the current substep is sampled.
+// Times the launches of its scope when the current substep is sampled.  Bracketing form (the
+// default): events recorded on the stream before and after the scope's launches, so the time
+// includes the dispatch gaps between them.  Dispatch form (`dispatch` true): the scope makes one
+// launch through DS_LAUNCH_T / DS_PPT_DISPATCH*_T, which hands both events to
+// hipExtLaunchKernel, and the time is the kernel's own execution.
 struct KernelTimer {
     ds_ctx* c;
     int id;
+    bool dispatch;
     hipEvent_t a = nullptr, b = nullptr;
-    KernelTimer(ds_ctx* c_, int id_) : c(c_), id(id_)
+    KernelTimer(ds_ctx* c_, int id_, bool dispatch_ = false) : c(c_), id(id_), dispatch(dispatch_)
     {
         if (!c->kt_on) return;
         a = kt_event(c);
         b = kt_event(c);
-        if (a && b) (void)hipEventRecord(a, c->stream);
+        if (!a || !b) {
+            if (a) c->kt_pool.push_back(a);
+            if (b) c->kt_pool.push_back(b);
+            a = b = nullptr;
+            return;
+        }
+        if (!dispatch) (void)hipEventRecord(a, c->stream);
     }
     ~KernelTimer()
     {
         if (!a || !b) return;
-        (void)hipEventRecord(b, c->stream);
+        if (!dispatch) (void)hipEventRecord(b, c->stream);
         c->kt_pending.push_back({id, {a, b}});
     }
 };
+
+// One launch, timed by a dispatch-form KernelTimer when it samples (never during capture: the
+// timer samples only outside it).
+#define DS_LAUNCH_T(kt, kern, grid, block, shm, stream, ...)                                     \
+    do {                                                                                         \
+        if ((kt).a) hipExtLaunchKernelGGL(kern, grid, block, shm, stream, (kt).a, (kt).b, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kern, grid, block, shm, stream, __VA_ARGS__);                    \
+    } while (0)
 
 // After a synchronisation: add the elapsed time of every recorded pair.
 void kt_collect(ds_ctx* c)
@@ -304,6 +335,17 @@ void drop_graphs(ds_ctx* c)
     if (c->fwd_exec) (void)hipGraphExecDestroy(c->fwd_exec);
     if (c->bwd_exec) (void)hipGraphExecDestroy(c->bwd_exec);
     c->fwd_exec = c->bwd_exec = nullptr;
+}
+
+void free_overlap(ds_ctx* c)
+{
+    dfree(c->d_seg_b);
+    dfree(c->d_seg_grid_b);
+    dfree(c->d_seg_aff_b);
+    dfree(c->d_seg_mj_b);
+    dfree(c->d_staging_r);
+    dfree(c->d_fbflag_r);
+    c->overlap_alloc = false;
 }
 
 void free_mode(ds_ctx* c)
@@ -352,6 +394,7 @@ void free_mode(ds_ctx* c)
     dfree(c->d_brows);
     dfree(c->d_rstart);
     dfree(c->d_src);
+    free_overlap(c);
     c->alloc_mode = -1;
 }
 
@@ -380,9 +423,14 @@ void free_all(ds_ctx* c)
     dfree(c->d_flag);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t* e : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_adj[0], &c->ev_adj[1]}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->ev0 = c->ev1 = nullptr;
-    c->stream = nullptr;
+    c->stream = c->stream2 = nullptr;
 }
 
 // Buffers that depend on the kernel family, allocated at the first ds_reset after a change.
@@ -460,6 +508,35 @@ int alloc_mode(ds_ctx* c)
     }
 #undef MALLOC
     c->alloc_mode = c->mode;
+    return DS_OK;
+}
+
+// The overlapped adjoint's second buffer set, staging, flags, stream and events (tiled family),
+// allocated at the first ds_backward that uses them.
+int alloc_overlap(ds_ctx* c)
+{
+    if (c->overlap_alloc || c->mode != DS_KERNELS_TILED) return DS_OK;
+    const long G = c->gridNodes() * c->B;
+    const size_t SF = (size_t)c->stateFloats();
+    int rc;
+#define MALLOC(ptr, n)                                  \
+    if ((rc = dalloc(c, &ptr, (size_t)(n))) != DS_OK) { \
+        free_overlap(c);                                \
+        return rc;                                      \
+    }
+    MALLOC(c->d_seg_b, SF * c->K);
+    MALLOC(c->d_seg_grid_b, (size_t)G * c->K);
+    MALLOC(c->d_seg_aff_b, (size_t)9 * c->P * c->K);
+    MALLOC(c->d_seg_mj_b, (size_t)10 * c->P * c->K);
+    MALLOC(c->d_staging_r, (size_t)c->staging_cap * TNODES);
+    MALLOC(c->d_fbflag_r, (size_t)c->B * c->NB * c->NB * c->NB);
+#undef MALLOC
+    fill_i32(c, c->d_fbflag_r, (long)c->B * c->NB * c->NB * c->NB, 0);
+    if (!c->stream2) HIPCHK(c, hipStreamCreate(&c->stream2));
+    for (hipEvent_t* e : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_adj[0], &c->ev_adj[1]})
+        if (!*e) HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->overlap_alloc = true;
     return DS_OK;
 }
 
@@ -598,62 +675,85 @@ inline dim3 chunk_grid(long maxchunks) { return dim3((unsigned)std::max<long>(1,
         else if ((ppt) == 2) hipLaunchKernelGGL((kern<2, b>), __VA_ARGS__);                      \
         else hipLaunchKernelGGL((kern<4, b>), __VA_ARGS__);                                      \
     } while (0)
+#define DS_PPT_DISPATCH_T(kt, ppt, kern, ...)                                                    \
+    do {                                                                                         \
+        if ((ppt) == 1) DS_LAUNCH_T(kt, kern<1>, __VA_ARGS__);                                   \
+        else if ((ppt) == 2) DS_LAUNCH_T(kt, kern<2>, __VA_ARGS__);                              \
+        else DS_LAUNCH_T(kt, kern<4>, __VA_ARGS__);                                              \
+    } while (0)
+#define DS_PPT_DISPATCH2_T(kt, ppt, kern, b, ...)                                                \
+    do {                                                                                         \
+        if ((ppt) == 1) DS_LAUNCH_T(kt, (kern<1, b>), __VA_ARGS__);                              \
+        else if ((ppt) == 2) DS_LAUNCH_T(kt, (kern<2, b>), __VA_ARGS__);                         \
+        else DS_LAUNCH_T(kt, (kern<4, b>), __VA_ARGS__);                                         \
+    } while (0)
+
+// Where the forward-type kernels (P2G, grid reduction, G2P, J) run: the stream and the scratch
+// they write besides their outputs.  The forward and a sequential recompute use the main stream's;
+// a recompute overlapped with the adjoint uses the second stream and its own staging and flags.
+struct Exec {
+    hipStream_t stream;
+    float4* staging;
+    int* fbflag;
+};
+inline Exec main_exec(const ds_ctx* c) { return Exec{c->stream, c->d_staging, c->d_fbflag}; }
 
 // P2G of a checkpoint segment's chunks into the staging slots, then the grid reduction:
 // velocity into d_vgrid and, when pout is given, momentum/mass into pout.
-void p2g_reduce_tiled(ds_ctx* c, const float* in, float* fout, int cp, float4* pout, float* aout)
+void p2g_reduce_tiled(ds_ctx* c, const Exec& x, const float* in, float* fout, int cp, float4* pout, float* aout)
 {
     // the scatter's chunk set: rows of distinct stencil bases, or packed gather chunks
     const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
     const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
     const int* counts = c->d_counts + 4 * cp;
     {
-        KernelTimer kt(c, DS_K_P2G);
+        KernelTimer kt(c, DS_K_P2G, true);
         if (c->packed)
-            DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_tiled, GChunk, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, in,
-                             fout, aout, c->d_gchunks + (size_t)cp * c->maxgch, nullptr, counts + 3, (int)c->maxgch,
-                             c->d_params, c->d_staging, c->d_pfb, c->d_fbflag, c->NB, c->d_j, c->dsc, c->P,
-                             c->d_flag);
+            DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_p2g_tiled, GChunk, chunk_grid(c->maxgch), dim3(WG), 0, x.stream,
+                               in, fout, aout, c->d_gchunks + (size_t)cp * c->maxgch, nullptr, counts + 3,
+                               (int)c->maxgch, c->d_params, x.staging, c->d_pfb, x.fbflag, c->NB, c->d_j, c->dsc,
+                               c->P, c->d_flag);
         else
-            DS_PPT_DISPATCH2(c->ppt_sorted, k_p2g_tiled, Chunk, chunk_grid(c->maxch), dim3(WG), 0, c->stream, in,
-                             fout, aout, c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1),
-                             counts + 1, (int)c->maxch, c->d_params, c->d_staging, c->d_pfb, c->d_fbflag, c->NB,
-                             c->d_j, c->dsc, c->P, c->d_flag);
+            DS_PPT_DISPATCH2_T(kt, c->ppt_sorted, k_p2g_tiled, Chunk, chunk_grid(c->maxch), dim3(WG), 0, x.stream, in,
+                               fout, aout, c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1),
+                               counts + 1, (int)c->maxch, c->d_params, x.staging, c->d_pfb, x.fbflag, c->NB,
+                               c->d_j, c->dsc, c->P, c->d_flag);
     }
     const int nb3 = c->NB * c->NB * c->NB;
-    KernelTimer kt(c, DS_K_GRID_REDUCE);
+    KernelTimer kt(c, DS_K_GRID_REDUCE, true);
     const int* cov = c->d_cov + (size_t)cp * c->B * nb3;
-    hipLaunchKernelGGL(k_grid_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart, cov,
-                       c->d_pfb, c->d_fbflag, pout, c->d_vgrid, c->dsc, c->NB);
+    DS_LAUNCH_T(kt, k_grid_reduce, dim3(c->B * nb3), dim3(64), 0, x.stream, x.staging, bnch, cstart, cov, c->d_pfb,
+                x.fbflag, pout, c->d_vgrid, c->dsc, c->NB);
 }
 
-void g2p_tiled(ds_ctx* c, const float* in, float* out, int cp, int f, bool record, double* mj = nullptr)
+void g2p_tiled(ds_ctx* c, const Exec& x, const float* in, float* out, int cp, int f, bool record,
+               double* mj = nullptr)
 {
-    KernelTimer kt(c, mj ? DS_K_G2P_FACTOR : DS_K_G2P);
+    KernelTimer kt(c, mj ? DS_K_G2P_FACTOR : DS_K_G2P, true);
     const GChunk* gch = c->d_gchunks + (size_t)cp * c->maxgch;
     const int* ngc = c->d_counts + 4 * cp + 3;
     double* dv = record ? c->d_dvsum : nullptr;
     if (mj)
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, true, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, in, out,
-                         gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P,
-                         c->d_flag);
+        DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_g2p_tiled, true, chunk_grid(c->maxgch), dim3(WG), 0, x.stream, in,
+                           out, gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc,
+                           c->P, c->d_flag);
     else
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_tiled, false, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, in, out,
-                         gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P,
-                         c->d_flag);
+        DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_g2p_tiled, false, chunk_grid(c->maxgch), dim3(WG), 0, x.stream, in,
+                           out, gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc,
+                           c->P, c->d_flag);
 }
 
 // J of every liquid particle of the checkpoint state `st` into d_j, for the segment's first P2G
 // (and, given mj, J and its SVD backward factor into mj).
-void liquid_j_tiled(ds_ctx* c, const float* st, int cp, double* mj = nullptr)
+void liquid_j_tiled(ds_ctx* c, const Exec& x, const float* st, int cp, double* mj = nullptr)
 {
     const GChunk* gch = c->d_gchunks + (size_t)cp * c->maxgch;
     const int* ngc = c->d_counts + 4 * cp + 3;
     if (mj)
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, true, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, st, gch,
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, true, chunk_grid(c->maxgch), dim3(WG), 0, x.stream, st, gch,
                          ngc, (int)c->maxgch, c->d_j, mj, c->dsc, c->P);
     else
-        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, false, chunk_grid(c->maxgch), dim3(WG), 0, c->stream, st, gch,
+        DS_PPT_DISPATCH2(c->gppt_sorted, k_liquid_j, false, chunk_grid(c->maxgch), dim3(WG), 0, x.stream, st, gch,
                          ngc, (int)c->maxgch, c->d_j, mj, c->dsc, c->P);
 }
 
@@ -671,95 +771,161 @@ int forward_tiled(ds_ctx* c, int nsteps)
                 RCHK(sort_into_checkpoint(c, cp));
             }
             in = c->d_ckpt + (size_t)cp * SF;
-            liquid_j_tiled(c, in, cp);
+            liquid_j_tiled(c, main_exec(c), in, cp);
         }
-        p2g_reduce_tiled(c, in, c->d_state, cp, nullptr, nullptr);
-        g2p_tiled(c, in, c->d_state, cp, s, true);
+        p2g_reduce_tiled(c, main_exec(c), in, c->d_state, cp, nullptr, nullptr);
+        g2p_tiled(c, main_exec(c), in, c->d_state, cp, s, true);
         c->step++;
     }
     return DS_OK;
 }
 
-int backward_tiled(ds_ctx* c)
+// One buffer set of a checkpoint segment: each step's state, momentum/mass grid, liquid affine,
+// and J with its SVD backward factor.
+struct SegSet {
+    float* seg;
+    float4* grid;
+    float* aff;
+    double* mj;
+};
+inline SegSet seg_set(const ds_ctx* c, int i)
 {
-    const int S = c->step;
+    return i == 0 ? SegSet{c->d_seg, c->d_seg_grid, c->d_seg_aff, c->d_seg_mj}
+                  : SegSet{c->d_seg_b, c->d_seg_grid_b, c->d_seg_aff_b, c->d_seg_mj_b};
+}
+
+// Substeps [s0, s1) of checkpoint segment cp (s1 = min(s0 + K, recorded steps)).
+inline void segment_range(const ds_ctx* c, int cp, int S, int& s0, int& L)
+{
+    s0 = cp * c->K;
+    L = std::min(s0 + c->K, S) - s0;
+}
+
+// Recompute checkpoint segment cp from its checkpoint into buffer set z, keeping each step's
+// state, momentum/mass grid, affine and J factor.
+void recompute_segment(ds_ctx* c, const Exec& x, int cp, int S, const SegSet& z)
+{
+    const long SF = c->stateFloats();
+    const long G = c->gridNodes() * c->B;
+    int s0, L;
+    segment_range(c, cp, S, s0, L);
+    const float* ck = c->d_ckpt + (size_t)cp * SF;
+    auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : z.seg + (size_t)k * SF; };
+    liquid_j_tiled(c, x, ck, cp, z.mj);
+    for (int k = 0; k < L; k++) {
+        const int s = s0 + k;
+        c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
+        p2g_reduce_tiled(c, x, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, z.grid + (size_t)k * G,
+                         z.aff + (size_t)k * 9 * c->P);
+        if (k + 1 < L) g2p_tiled(c, x, st_of(k), st_of(k + 1), cp, s, false, z.mj + (size_t)(k + 1) * 10 * c->P);
+    }
+}
+
+// Adjoint of checkpoint segment cp from its recomputed buffer set z, last step first, on the
+// main stream; then the segment's parameter-gradient partials into d_thbar and the particle
+// adjoint back into the previous checkpoint's order.
+void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
+{
     const long SF = c->stateFloats();
     const long G = c->gridNodes() * c->B;
     const int nb3 = c->NB * c->NB * c->NB;
     const dim3 rgrid = chunk_grid(c->maxch), ggrid = chunk_grid(c->maxgch);
     const int maxch = (int)c->maxch, maxgch = (int)c->maxgch;
-    for (int cp = (S - 1) / c->K; cp >= 0; cp--) {
-        const int s0 = cp * c->K;
-        const int s1 = std::min(s0 + c->K, S);
-        const int L = s1 - s0;
-        const int* counts = c->d_counts + 4 * cp;   // particles, row chunks, rows, gather chunks
-        const Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
-        const int* rows = c->d_rows + (size_t)cp * (c->P + 1);
-        const GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
-        const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
-        const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
-        const float* ck = c->d_ckpt + (size_t)cp * SF;
-        auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : c->d_seg + (size_t)k * SF; };
-        auto grid_of = [&](int k) -> float4* { return c->d_seg_grid + (size_t)k * G; };
-        auto aff_of = [&](int k) -> float* { return c->d_seg_aff + (size_t)k * 9 * c->P; };
-        auto mj_of = [&](int k) -> double* { return c->d_seg_mj + (size_t)k * 10 * c->P; };
-        // recompute the segment, keeping each step's state and momentum/mass grid
-        fill_zero_f64(c, c->d_thpart, 4L * c->maxgch);
-        liquid_j_tiled(c, ck, cp, mj_of(0));
-        for (int k = 0; k < L; k++) {
-            const int s = s0 + k;
-            c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
-            p2g_reduce_tiled(c, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, grid_of(k), aff_of(k));
-            if (k + 1 < L) g2p_tiled(c, st_of(k), st_of(k + 1), cp, s, false, mj_of(k + 1));
+    int s0, L;
+    segment_range(c, cp, S, s0, L);
+    const int* counts = c->d_counts + 4 * cp;   // particles, row chunks, rows, gather chunks
+    const Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
+    const int* rows = c->d_rows + (size_t)cp * (c->P + 1);
+    const GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
+    const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
+    const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
+    const float* ck = c->d_ckpt + (size_t)cp * SF;
+    auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : z.seg + (size_t)k * SF; };
+    auto grid_of = [&](int k) -> float4* { return z.grid + (size_t)k * G; };
+    auto aff_of = [&](int k) -> float* { return z.aff + (size_t)k * 9 * c->P; };
+    auto mj_of = [&](int k) -> double* { return z.mj + (size_t)k * 10 * c->P; };
+    fill_zero_f64(c, c->d_thpart, 4L * c->maxgch);
+    for (int k = L - 1; k >= 0; k--) {
+        const int s = s0 + k;
+        c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
+        {
+            KernelTimer kt(c, DS_K_G2P_ADJ, true);
+            if (c->packed)
+                DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_g2p_adj_tiled, GChunk, ggrid, dim3(WG), 0, c->stream,
+                                   st_of(k), c->d_adj, gchunks, nullptr, counts + 3, maxgch, c->d_dvbar, c->T, s,
+                                   c->d_staging, c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
+            else
+                DS_PPT_DISPATCH2_T(kt, c->ppt_sorted, k_g2p_adj_tiled, Chunk, rgrid, dim3(WG), 0, c->stream, st_of(k),
+                                   c->d_adj, chunks, rows, counts + 1, maxch, c->d_dvbar, c->T, s, c->d_staging,
+                                   c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
         }
-        // adjoint, last step first
-        for (int k = L - 1; k >= 0; k--) {
-            const int s = s0 + k;
-            c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
-            {
-                KernelTimer kt(c, DS_K_G2P_ADJ);
-                if (c->packed)
-                    DS_PPT_DISPATCH2(c->gppt_sorted, k_g2p_adj_tiled, GChunk, ggrid, dim3(WG), 0, c->stream, st_of(k),
-                                     c->d_adj, gchunks, nullptr, counts + 3, maxgch, c->d_dvbar, c->T, s,
-                                     c->d_staging, c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
-                else
-                    DS_PPT_DISPATCH2(c->ppt_sorted, k_g2p_adj_tiled, Chunk, rgrid, dim3(WG), 0, c->stream, st_of(k),
-                                     c->d_adj, chunks, rows, counts + 1, maxch, c->d_dvbar, c->T, s, c->d_staging,
-                                     c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
-            }
-            {
-                KernelTimer kt(c, DS_K_VBAR_REDUCE);
-                hipLaunchKernelGGL(k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
-                                   c->d_cov + (size_t)cp * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2,
-                                   c->dsc, c->NB);
-            }
-            if (c->adj_split) {
-                {
-                    KernelTimer kt(c, DS_K_P2G_ADJ);
-                    DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, false, ggrid, dim3(WG), 0, c->stream, st_of(k),
-                                     aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2,
-                                     c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
-                }
-                KernelTimer kt(c, DS_K_P2G_ADJ_STRESS);
-                DS_PPT_DISPATCH(c->gppt_sorted, k_p2g_adj_stress, ggrid, dim3(WG), 0, c->stream, st_of(k), c->d_adj,
-                                gchunks, counts + 3, maxgch, c->d_params, c->d_gadj, mj_of(k), c->d_thpart, c->dsc,
-                                c->P);
-            } else {
-                KernelTimer kt(c, DS_K_P2G_ADJ);
-                DS_PPT_DISPATCH2(c->gppt_sorted, k_p2g_adj_tiled, true, ggrid, dim3(WG), 0, c->stream, st_of(k),
-                                 aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2, c->d_gadj,
-                                 c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
-            }
+        {
+            KernelTimer kt(c, DS_K_VBAR_REDUCE, true);
+            DS_LAUNCH_T(kt, k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
+                        c->d_cov + (size_t)cp * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2, c->dsc,
+                        c->NB);
         }
-        hipLaunchKernelGGL(k_thpart_reduce, dim3((unsigned)std::min<long>((c->maxgch + 255) / 256, 1024)), dim3(256),
-                           0, c->stream, gchunks, counts + 3, maxgch, c->d_thpart, c->d_thbar);
+        if (c->adj_split) {
+            {
+                KernelTimer kt(c, DS_K_P2G_ADJ, true);
+                DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_p2g_adj_tiled, false, ggrid, dim3(WG), 0, c->stream,
+                                   st_of(k), aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k),
+                                   c->d_pbar2, c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
+            }
+            KernelTimer kt(c, DS_K_P2G_ADJ_STRESS, true);
+            DS_PPT_DISPATCH_T(kt, c->gppt_sorted, k_p2g_adj_stress, ggrid, dim3(WG), 0, c->stream, st_of(k), c->d_adj,
+                              gchunks, counts + 3, maxgch, c->d_params, c->d_gadj, mj_of(k), c->d_thpart, c->dsc,
+                              c->P);
+        } else {
+            KernelTimer kt(c, DS_K_P2G_ADJ, true);
+            DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_p2g_adj_tiled, true, ggrid, dim3(WG), 0, c->stream, st_of(k),
+                               aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2, c->d_gadj,
+                               c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
+        }
+    }
+    hipLaunchKernelGGL(k_thpart_reduce, dim3((unsigned)std::min<long>((c->maxgch + 255) / 256, 1024)), dim3(256), 0,
+                       c->stream, gchunks, counts + 3, maxgch, c->d_thpart, c->d_thbar);
+    if (cp > 0) {
+        // the adjoint in the previous checkpoint's particle order, back into the fixed buffer
+        hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
+                           c->d_src + (size_t)cp * c->P, c->nreal, c->P);
+        hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)std::min<long>(nblocks(SF), 8192)), dim3(256), 0, c->stream,
+                           c->d_adj, c->d_adj2, SF);
+    }
+}
+
+// The checkpointed adjoint of the recorded rollout.  Sequential: recompute segment k, then its
+// adjoint, for k from the last segment down.  Overlapped: segment k-1 is recomputed on the
+// second stream (into the other buffer set, with its own staging and flags) while segment k's
+// adjoint runs on the main stream; events order each recompute after the adjoint that last read
+// its buffer set, and each adjoint after its recompute.  Both orders run the same kernels on
+// the same data, so the results are identical.
+int backward_tiled(ds_ctx* c)
+{
+    const int S = c->step;
+    const int last = (S - 1) / c->K;
+    if (!c->overlap || !c->overlap_alloc) {
+        for (int cp = last; cp >= 0; cp--) {
+            recompute_segment(c, main_exec(c), cp, S, seg_set(c, 0));
+            adjoint_segment(c, cp, S, seg_set(c, 0));
+        }
+        return DS_OK;
+    }
+    const Exec xr{c->stream2, c->d_staging_r, c->d_fbflag_r};
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    recompute_segment(c, xr, last, S, seg_set(c, last & 1));
+    HIPCHK(c, hipEventRecord(c->ev_rec[last & 1], c->stream2));
+    for (int cp = last; cp >= 0; cp--) {
         if (cp > 0) {
-            // the adjoint in the previous checkpoint's particle order, back into the fixed buffer
-            hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
-                               c->d_src + (size_t)cp * c->P, c->nreal, c->P);
-            hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)std::min<long>(nblocks(SF), 8192)), dim3(256), 0, c->stream,
-                               c->d_adj, c->d_adj2, SF);
+            // buffer set (cp - 1) & 1 was last read by the adjoint of segment cp + 1
+            if (cp + 1 <= last) HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_adj[(cp + 1) & 1], 0));
+            recompute_segment(c, xr, cp - 1, S, seg_set(c, (cp - 1) & 1));
+            HIPCHK(c, hipEventRecord(c->ev_rec[(cp - 1) & 1], c->stream2));
         }
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_rec[cp & 1], 0));
+        adjoint_segment(c, cp, S, seg_set(c, cp & 1));
+        HIPCHK(c, hipEventRecord(c->ev_adj[cp & 1], c->stream));
     }
     return DS_OK;
 }
@@ -944,6 +1110,12 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         c->packed = value == DS_SCATTER_PACKED;
         return DS_OK;
     }
+    if (option == DS_OPT_OVERLAP) {
+        if (value != 0 && value != 1) return fail(c, DS_EINVAL, "DS_OPT_OVERLAP takes 0 or 1");
+        if (c->overlap != (value == 1)) drop_graphs(c);
+        c->overlap = value == 1;
+        return DS_OK;
+    }
     if (option == DS_OPT_ADJOINT_P2G) {
         if (value != DS_ADJ_P2G_FUSED && value != DS_ADJ_P2G_SPLIT)
             return fail(c, DS_EINVAL, "DS_OPT_ADJOINT_P2G takes DS_ADJ_P2G_FUSED or DS_ADJ_P2G_SPLIT");
@@ -1101,6 +1273,7 @@ int ds_backward(ds_ctx* c, const double* dl_dforces, double* dl_dparams)
     const double scale = c->desc.m_stick / c->desc.dt;
     for (size_t i = 0; i < dvb.size(); i++) dvb[i] = scale * dl_dforces[i];
     HIPCHK(c, hipMemcpyAsync(c->d_dvbar, dvb.data(), sizeof(double) * dvb.size(), hipMemcpyHostToDevice, c->stream));
+    if (c->mode == DS_KERNELS_TILED && c->overlap) RCHK(alloc_overlap(c));
     const bool graph = c->use_graph && c->mode == DS_KERNELS_TILED && c->step == c->T - 1;
     if (graph && !c->bwd_exec) RCHK(capture_graph(c, backward_tiled_n, 0, &c->bwd_exec));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -46,13 +46,14 @@ class Engine:
 
     # execution options of the tiled kernels (ds_set_option ids)
     OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
+    OPT_OVERLAP = 8
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
     def __init__(self, scenes: Sequence[Scene], ckpt_every: int = 50, max_steps: Optional[int] = None,
                  kernels: str = "tiled", particles_per_thread: int = 2, kernel_timing_every: int = 0,
                  gather_particles_per_thread: int = 1, adjoint_p2g: str = "fused", scatter_chunks: str = "rows",
-                 graph: bool = False):
+                 graph: bool = False, overlap: bool = True):
         self.lib = _lib.load()
         scenes = list(scenes)
         if not scenes:
@@ -81,7 +82,7 @@ class Engine:
         self._check(self.lib.ds_set_option(self.ctx, self.OPT_TIMING, int(kernel_timing_every)))
         self.configure(particles_per_thread=particles_per_thread,
                        gather_particles_per_thread=gather_particles_per_thread, adjoint_p2g=adjoint_p2g,
-                       scatter_chunks=scatter_chunks)
+                       scatter_chunks=scatter_chunks, overlap=overlap)
         self.set_graph(graph)
         for b, s in enumerate(scenes):
             self.upload_scene(b, s)
@@ -102,9 +103,12 @@ class Engine:
         self._check(self.lib.ds_set_trajectory(self.ctx, b, _p32(tr)))
 
     def configure(self, particles_per_thread: Optional[int] = None, gather_particles_per_thread: Optional[int] = None,
-                  adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None):
+                  adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None,
+                  overlap: Optional[bool] = None):
         """Execution options of the tiled kernels (they change how the substep runs, never what it
-        computes); particle-per-thread changes take effect at the next reset."""
+        computes); particle-per-thread changes take effect at the next reset.  `overlap`: the
+        adjoint recomputes the next checkpoint segment on a second stream while the current one's
+        adjoint runs (a second set of segment buffers)."""
         if particles_per_thread is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_PPT, int(particles_per_thread)))
             self.particles_per_thread = int(particles_per_thread)
@@ -121,6 +125,9 @@ class Engine:
                 raise ValueError(f"scatter_chunks must be one of {list(self.SCATTER)}")
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_SCATTER, self.SCATTER[scatter_chunks]))
             self.scatter_chunks = scatter_chunks
+        if overlap is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_OVERLAP, 1 if overlap else 0))
+            self.overlap = bool(overlap)
 
     def set_kernel_timing(self, every: int):
         """Sample every N-th substep's kernels with HIP events (0: off)."""
@@ -135,7 +142,7 @@ class Engine:
     def options(self) -> dict:
         return {"particles_per_thread": self.particles_per_thread,
                 "gather_particles_per_thread": self.gather_particles_per_thread, "adjoint_p2g": self.adjoint_p2g,
-                "scatter_chunks": self.scatter_chunks}
+                "scatter_chunks": self.scatter_chunks, "overlap": self.overlap}
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):

@@ -52,6 +52,9 @@ enum {
     DS_OPT_GRAPH = 7,                  /* 1: a whole-rollout ds_forward (from step 0 to the end) and the
                                           ds_backward after it replay hipGraphs captured at their first
                                           call (no per-kernel timing inside them); 0 (default): launches */
+    DS_OPT_OVERLAP = 8,                /* tiled adjoint: 1 (default) recomputes segment k-1 on a second
+                                          stream while segment k's adjoint runs (a second segment buffer
+                                          set, allocated at the first ds_backward); 0: one stream */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or

@@ -194,6 +194,31 @@ def test_graph_replay_matches_launches():
     assert not np.array_equal(out[True][0][0], out[True][1][0])   # the parameters did change
 
 
+def test_overlapped_adjoint_matches_one_stream():
+    """The adjoint with segment k-1 recomputed on a second stream during segment k's adjoint (two
+    segment buffer sets, separate staging) gives the one-stream results, launched and as a replayed
+    hipGraph, over an odd and an even number of checkpoint segments and several calls in a row."""
+    scs = [S.tiny_scene(16, 40, seed=8), S.tiny_scene(16, 40, seed=9)]
+    for K in (6, 13):   # 7 and 3 segments
+        out = {}
+        for overlap, graph in ((False, False), (True, False), (True, True)):
+            print(f"[gpu] overlap {overlap}, graph {graph}, K {K}", flush=True)
+            with Engine(scs, ckpt_every=K, overlap=overlap, graph=graph) as e:
+                res = []
+                for scale in (1.0, 0.8, 1.0):
+                    e.set_params(np.stack([s.params * np.array([scale, 1.0, 1.0, 1.0]) for s in scs]))
+                    e.reset()
+                    e.forward()
+                    F = e.forces()
+                    res.append((F.copy(), e.backward(np.ones_like(F)).copy()))
+                out[(overlap, graph)] = res
+        ref = out[(False, False)]
+        for key in ((True, False), (True, True)):
+            for (Fa, da), (Fb, db) in zip(ref, out[key]):
+                np.testing.assert_allclose(Fb, Fa, rtol=1e-10, atol=1e-10 * np.abs(Fa).max())
+                np.testing.assert_allclose(db, da, rtol=1e-10, atol=1e-10 * np.abs(da).max(), err_msg=str((K, key)))
+
+
 def test_autotune_picks_a_candidate_on_the_device():
     from diffstir_amd import autotune as A
     A.clear_cache()
