@@ -173,21 +173,50 @@ def plain_single(workload: str, sample_steps: int, ckpt: int) -> dict:
             "fwd_adjoint_particle_steps_per_s": sc.n_liquid * sample_steps / el}
 
 
-def cpu_baseline(workload: str, sample_steps: int, ckpt: int):
+def cpu_baseline(workload: str, sample_steps: int, ckpt: int, check_workloads=()):
     """The FP64 oracle, one thread, on the first `sample_steps` substeps of the workload's first
-    instance (forward + checkpointed adjoint)."""
+    instance (forward + checkpointed adjoint).  The oracle's forces and parameter gradient on
+    that sample also check the GPU: the tiled kernels run the same sample and their relative
+    errors are reported, for `workload` and for instance 0 of each (workload, substeps) of
+    `check_workloads`."""
     from oracle.oracle import Oracle, loss_and_grad
     o = Oracle("f64")
     sc = S.workload(workload)[0].truncated(sample_steps + 1)
     t0 = time.perf_counter()
     F = o.rollout(sc, nsteps=sample_steps)
     L, g = loss_and_grad(F, 0.5 * F, sc.cfg.loss_kind)
-    o.rollout_grad(sc, g, nsteps=sample_steps, ckpt=min(ckpt, sample_steps))
+    _, dth = o.rollout_grad(sc, g, nsteps=sample_steps, ckpt=min(ckpt, sample_steps))
     el = time.perf_counter() - t0
     # the timed region ran the forward twice (plain + inside rollout_grad); count one fwd+adj
-    return dict(value=sc.n_liquid * sample_steps / el, unit="fwd+adjoint particle-steps/s", cores=1, kind="port",
-                sample=f"{workload} instance 0, first {sample_steps} substeps, forward + checkpointed adjoint "
-                       f"(plus one extra forward), FP64 oracle, {el:.1f} s")
+    out = dict(value=sc.n_liquid * sample_steps / el, unit="fwd+adjoint particle-steps/s", cores=1, kind="port",
+               sample=f"{workload} instance 0, first {sample_steps} substeps, forward + checkpointed adjoint "
+                      f"(plus one extra forward), FP64 oracle, {el:.1f} s")
+    checks = {workload: gpu_vs_oracle(sc, F, dth, ckpt)}
+    for wl, n in check_workloads:
+        sb = S.workload(wl)[0].truncated(n + 1)
+        Fb = o.rollout(sb, nsteps=n)
+        _, gb = loss_and_grad(Fb, 0.5 * Fb, sb.cfg.loss_kind)
+        _, db = o.rollout_grad(sb, gb, nsteps=n, ckpt=min(ckpt, n))
+        checks[wl] = gpu_vs_oracle(sb, Fb, db, ckpt)
+    out["gpu_vs_oracle"] = checks
+    return out
+
+
+def gpu_vs_oracle(sc, F_o, dth_o, ckpt) -> dict:
+    """Relative errors of the GPU's stick forces and parameter gradient against the FP64 oracle's
+    on the same truncated scene, with the same loss (target: half the oracle's forces)."""
+    from diffstir_amd.loss import compute_loss
+    T = sc.max_steps
+    with Engine([sc], ckpt_every=min(ckpt, T - 1)) as e:
+        e.reset()
+        e.forward()
+        Fg = e.forces()[0]
+        _, gg = compute_loss(Fg, 0.5 * F_o, sc.cfg.loss_kind)
+        dg = e.backward(gg[None])[0]
+    fe = float(np.abs(Fg[2:] - F_o[2:]).max() / np.abs(F_o[2:]).max())
+    ge = float(np.abs(dg - dth_o).max() / np.abs(dth_o).max())
+    return {"substeps": T - 1, "liquid_particles": sc.n_liquid, "force_rel_err": fe, "grad_rel_err": ge,
+            "tolerance": {"force": 1e-4, "grad": 1e-3}}
 
 
 def roofline(r) -> dict:
@@ -306,7 +335,11 @@ def main():
         line["extra"]["batched"]["ratio_to_plain_single_instance_evals"] = (
             line["extra"]["batched"]["gradient_evals_per_s"] / plain["gradient_evals_per_s"])
     if not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_sample_steps, a.ckpt)
+        # the other workloads' first instances on short samples: the batched sweep, the
+        # identification scene (Herschel-Bulkley, 3000 substeps) and the n_grid 128 instance
+        checks = () if a.no_batch else ((a.batch_workload, a.cpu_sample_steps), ("ident16_n64", a.cpu_sample_steps),
+                                        ("large_n128", 4))
+        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_sample_steps, a.ckpt, check_workloads=checks)
     print(json.dumps(line), flush=True)
 
 

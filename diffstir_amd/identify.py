@@ -89,6 +89,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--noise", type=float, default=0.01)
     ap.add_argument("--ckpt", type=int, default=50)
+    ap.add_argument("--out", default="", help="also write the loss curve and parameter history (JSON)")
     a = ap.parse_args(argv)
     scenes = S.workload(a.workload)
     truth = np.asarray(scenes[0].params, dtype=np.float64)
@@ -98,17 +99,22 @@ def main(argv=None):
     inits[:, :3] = np.maximum(inits[:, :3], 1e-2)
     t0 = time.perf_counter()
     res = identify(scenes, tgt, inits, iterations=a.iterations, lr=a.lr, ckpt_every=a.ckpt,
-                   callback=lambda it, L, th: print(f"[identify] iteration {it}: loss min {np.min(L):.4g}",
-                                                    flush=True))
+                   callback=lambda it, L, th: print(f"[identify] iteration {it}: loss min {np.min(L):.4g} "
+                                                    f"median {np.median(L):.4g}", flush=True))
     el = time.perf_counter() - t0
     best = int(np.argmin(res["losses"][-1]))
     fin = res["params"][-1][best]
-    print(json.dumps({"workload": a.workload, "restarts": len(scenes), "iterations": a.iterations,
-                      "substeps": scenes[0].max_steps - 1, "s_per_iteration": el / a.iterations,
-                      "gradient_evals_per_s": len(scenes) * a.iterations / el,
-                      "loss_first_min": float(np.min(res["losses"][0])), "loss_last_min": float(np.min(res["losses"][-1])),
-                      "truth": truth[:3].tolist(), "best_restart": fin[:3].tolist(),
-                      "relative_error": (np.abs(fin[:3] - truth[:3]) / truth[:3]).tolist()}))
+    summary = {"workload": a.workload, "restarts": len(scenes), "iterations": a.iterations,
+               "substeps": scenes[0].max_steps - 1, "s_per_iteration": el / a.iterations,
+               "gradient_evals_per_s": len(scenes) * a.iterations / el,
+               "loss_first_min": float(np.min(res["losses"][0])), "loss_last_min": float(np.min(res["losses"][-1])),
+               "truth": truth[:3].tolist(), "best_restart": fin[:3].tolist(),
+               "relative_error": (np.abs(fin[:3] - truth[:3]) / truth[:3]).tolist()}
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(dict(summary, losses=res["losses"].tolist(), params=res["params"].tolist(),
+                           inits=inits.tolist()), f)
+    print(json.dumps(summary))
 
 
 if __name__ == "__main__":
