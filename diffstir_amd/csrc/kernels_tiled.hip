@@ -1030,51 +1030,77 @@ __global__ void __launch_bounds__(256) k_bucket_totals(const int* __restrict__ c
     }
 }
 
-// Exclusive scans of four int arrays a[k] by one workgroup of 1024 threads into o[k]; tot[k]
-// gets the sums.
+// Exclusive scans of four int arrays a[k] into o[k] (tot[k] gets the sums), in three passes
+// over tiles of 1024 elements: the tile sums (k_scan4_sums), a scan of the tile sums in place by
+// one workgroup (k_scan4_tiles), and the scan within each tile plus its offset (k_scan4_apply).
 struct Scan4 {
     const int* a[4];
     int* o[4];
 };
+constexpr int SCAN_TILE = 1024;
 
-__global__ void __launch_bounds__(1024) k_scan4(Scan4 q, int n, int* __restrict__ tot)
+// Inclusive scan of one value per thread across a 1024-thread workgroup (ws: 17 ints of LDS);
+// `total` gets the workgroup's sum.
+__device__ __forceinline__ int block_incl_scan(int v, int* ws, int& total)
 {
-    __shared__ int ws[4][16];
-    __shared__ int carry[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (threadIdx.x < 4) carry[threadIdx.x] = 0;
+    const int sc = wave_incl_scan(v);
+    if (lane == 63) ws[w] = sc;
     __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + threadIdx.x;
-        int v[4], sc[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            v[k] = i < n ? q.a[k][i] : 0;
-            sc[k] = wave_incl_scan(v[k]);
-        }
-        if (lane == 63) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) ws[k][w] = sc[k];
-        }
-        __syncthreads();
-        if (threadIdx.x < 4) {
-            const int k = threadIdx.x;
-            int run = carry[k];
-            for (int j = 0; j < 16; j++) {
-                const int tj = ws[k][j];
-                ws[k][j] = run;
-                run += tj;
-            }
-            carry[k] = run;
-        }
-        __syncthreads();
-        if (i < n) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) q.o[k][i] = ws[k][w] + sc[k] - v[k];
-        }
-        __syncthreads();
+    if (threadIdx.x < 64) {
+        const int x = threadIdx.x < 16 ? ws[threadIdx.x] : 0;
+        const int y = wave_incl_scan(x);
+        if (threadIdx.x < 16) ws[threadIdx.x] = y - x;   // exclusive wave offsets
+        if (threadIdx.x == 15) ws[16] = y;
     }
-    if (threadIdx.x < 4) tot[threadIdx.x] = carry[threadIdx.x];
+    __syncthreads();
+    const int r = ws[w] + sc;
+    total = ws[16];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(SCAN_TILE) k_scan4_sums(Scan4 q, int n, int* __restrict__ tsum)
+{
+    __shared__ int ws[17];
+    const long i = (long)blockIdx.x * SCAN_TILE + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int v = i < n ? q.a[k][i] : 0;
+        int total;
+        block_incl_scan(v, ws, total);
+        if (threadIdx.x == 0) tsum[(long)k * gridDim.x + blockIdx.x] = total;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_TILE) k_scan4_tiles(int* __restrict__ tsum, int ntiles, int* __restrict__ tot)
+{
+    __shared__ int ws[17];
+    for (int k = 0; k < 4; k++) {
+        int carry = 0;
+        for (int base = 0; base < ntiles; base += SCAN_TILE) {
+            const int i = base + threadIdx.x;
+            const int v = i < ntiles ? tsum[(long)k * ntiles + i] : 0;
+            int total;
+            const int r = block_incl_scan(v, ws, total);
+            if (i < ntiles) tsum[(long)k * ntiles + i] = carry + r - v;
+            carry += total;
+        }
+        if (threadIdx.x == 0) tot[k] = carry;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_TILE) k_scan4_apply(Scan4 q, int n, const int* __restrict__ tsum)
+{
+    __shared__ int ws[17];
+    const long i = (long)blockIdx.x * SCAN_TILE + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int v = i < n ? q.a[k][i] : 0;
+        int total;
+        const int r = block_incl_scan(v, ws, total);
+        if (i < n) q.o[k][i] = tsum[(long)k * gridDim.x + blockIdx.x] + r - v;
+    }
 }
 
 // One wavefront per bucket: the round-robin slot of every (cell, rank) pair (slot = the

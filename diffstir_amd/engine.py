@@ -46,7 +46,7 @@ class Engine:
 
     # execution options of the tiled kernels (ds_set_option ids)
     OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
-    OPT_OVERLAP = 8
+    OPT_OVERLAP, OPT_CHUNK_GRID = 8, 9
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
@@ -104,11 +104,12 @@ class Engine:
 
     def configure(self, particles_per_thread: Optional[int] = None, gather_particles_per_thread: Optional[int] = None,
                   adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None,
-                  overlap: Optional[bool] = None):
+                  overlap: Optional[bool] = None, chunk_grid: Optional[int] = None):
         """Execution options of the tiled kernels (they change how the substep runs, never what it
         computes); particle-per-thread changes take effect at the next reset.  `overlap`: the
         adjoint recomputes the next checkpoint segment on a second stream while the current one's
-        adjoint runs (a second set of segment buffers)."""
+        adjoint runs (a second set of segment buffers).  `chunk_grid`: workgroups per chunk-kernel
+        launch (0: the library default)."""
         if particles_per_thread is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_PPT, int(particles_per_thread)))
             self.particles_per_thread = int(particles_per_thread)
@@ -128,6 +129,8 @@ class Engine:
         if overlap is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_OVERLAP, 1 if overlap else 0))
             self.overlap = bool(overlap)
+        if chunk_grid is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_CHUNK_GRID, int(chunk_grid)))
 
     def set_kernel_timing(self, every: int):
         """Sample every N-th substep's kernels with HIP events (0: off)."""

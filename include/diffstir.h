@@ -55,6 +55,9 @@ enum {
     DS_OPT_OVERLAP = 8,                /* tiled adjoint: 1 (default) recomputes segment k-1 on a second
                                           stream while segment k's adjoint runs (a second segment buffer
                                           set, allocated at the first ds_backward); 0: one stream */
+    DS_OPT_CHUNK_GRID = 9,             /* workgroups of each tiled chunk kernel launch (the chunk count lives
+                                          on the device; chunks beyond the grid loop): 0 = 16384 (default),
+                                          or 64..1048576 */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or

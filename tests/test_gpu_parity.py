@@ -219,6 +219,24 @@ def test_overlapped_adjoint_matches_one_stream():
                 np.testing.assert_allclose(db, da, rtol=1e-10, atol=1e-10 * np.abs(da).max(), err_msg=str((K, key)))
 
 
+def test_chunk_grid_does_not_change_results():
+    """The chunk kernels give the same forces and gradient whatever their launch grid: one
+    workgroup per chunk, or fewer workgroups each looping over several chunks."""
+    sc = S.tiny_scene(16, 20, seed=3)
+    out = []
+    with Engine([sc], ckpt_every=6) as e:
+        for grid in (0, 64, 1 << 20):
+            print(f"[gpu] chunk grid {grid}", flush=True)
+            e.configure(chunk_grid=grid)
+            e.reset()
+            e.forward()
+            F = e.forces()
+            out.append((F.copy(), e.backward(np.ones_like(F)).copy()))
+    for F, d in out[1:]:
+        np.testing.assert_allclose(F, out[0][0], rtol=1e-10, atol=1e-10 * np.abs(out[0][0]).max())
+        np.testing.assert_allclose(d, out[0][1], rtol=1e-10, atol=1e-10 * np.abs(out[0][1]).max())
+
+
 def test_autotune_picks_a_candidate_on_the_device():
     from diffstir_amd import autotune as A
     A.clear_cache()
