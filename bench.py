@@ -21,6 +21,7 @@ import json
 import os
 import sys
 import time
+from typing import Optional
 
 import numpy as np
 
@@ -92,7 +93,7 @@ def device_free_bytes() -> int:
 
 def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10,
             autotune: bool = True, graph: bool = True, kernel_sample: bool = True, substeps: int = 0,
-            overlap: bool = True):
+            overlap: bool = True, sort_every: Optional[int] = None):
     scenes = S.workload(workload)
     if substeps:
         # a shortened rollout (profiling runs whose every launch is serialised by the profiler)
@@ -110,6 +111,8 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
         t0 = time.perf_counter()
         tuned = tune(scenes, ckpt_every=ckpt, engine=eng)
         log(f"[bench] autotuned in {time.perf_counter() - t0:.1f} s: {tuned['options']}")
+    if sort_every is not None:
+        eng.configure(sort_every=sort_every)
     graph = graph and kernels == "tiled"
     eng.set_graph(graph)
     targets = make_targets(eng, scenes)
@@ -283,6 +286,9 @@ def main():
                     help="run the adjoint's segment recompute on the main stream (no second stream)")
     ap.add_argument("--substeps", type=int, default=0,
                     help="shorten the rollout to this many substeps (profiling only; 0 = the workload's own)")
+    ap.add_argument("--sort-every", type=int, default=None,
+                    help="sort the particles every this many substeps (a divisor of --ckpt; 0 = once per "
+                         "checkpoint interval); overrides the autotuner's choice")
     a = ap.parse_args()
     if a.calibrate_copy_mib:
         calibrate_copy(a.calibrate_copy_mib)
@@ -292,7 +298,7 @@ def main():
               "and nothing here uses a second GPU", file=sys.stderr)
         sys.exit(2)
     r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph,
-                substeps=a.substeps, overlap=not a.no_overlap)
+                substeps=a.substeps, overlap=not a.no_overlap, sort_every=a.sort_every)
     value = r["psteps"] / r["elapsed"]
     ms_per_step = 1000.0 * r["elapsed"] / a.steps
     bpps = fwd_adj_bytes(a.ckpt)

@@ -13,6 +13,10 @@ The options change how a substep runs on the device, never what it computes:
   ("packed": full wavefronts, lanes sharing a base served in successive passes).  With packed
   chunks the rows-per-wavefront choice does not apply.
 
+A second stage then tries sorting the particles into buckets more often than once per checkpoint
+interval (``sort_every``: half or a quarter/fifth of it): where particles drift far within an
+interval (the n_grid 128 workload) it keeps their stencils inside their chunk's tile.
+
 Which choice is fastest depends on the workload's size and on how the particles cluster, so each
 candidate is timed on the workload itself: a short checkpointed forward + adjoint rollout
 (``substeps`` substeps of every instance) on one Engine, best of ``repeats``.  The particles of a
@@ -57,6 +61,13 @@ def candidates(space: Optional[dict] = None) -> List[dict]:
             seen.add(key)
             out.append(d)
     return out
+
+
+def sort_candidates(ckpt_every: int) -> List[int]:
+    """Sort intervals of the second stage: the checkpoint interval itself (0) and up to two of its
+    divisors K/2, K/4, K/5."""
+    K = int(ckpt_every)
+    return [0] + [K // d for d in (2, 4, 5) if K % d == 0 and K // d >= 2][:2]
 
 
 def signature(scenes, ckpt_every: int) -> str:
@@ -148,6 +159,16 @@ def tune(scenes, ckpt_every: int = 50, substeps: Optional[int] = None, space: Op
             if verbose:
                 print(f"[autotune] {cand}: {t:.2f} ms", flush=True)
         best = pick(timings)
+        # second stage: the sort interval, with the chosen options
+        stage2 = [(dict(best, sort_every=0), min(t for c, t in timings if c == best))]
+        for R in sort_candidates(ckpt_every)[1:]:
+            cand = dict(best, sort_every=R)
+            t = timer(eng, cand, substeps, repeats)
+            stage2.append((cand, t))
+            timings.append((cand, t))
+            if verbose:
+                print(f"[autotune] {cand}: {t:.2f} ms", flush=True)
+        best = pick(stage2)
         if warm_steps > 0:
             for b, s in enumerate(scenes):
                 eng.upload_scene(b, s)

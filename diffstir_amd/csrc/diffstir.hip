@@ -37,6 +37,8 @@ struct ds_ctx {
     ds_scene_desc desc;
     DevScene dsc;
     int B = 0, T = 0, K = 0, ncp = 0;
+    int sort_every = 0;                // substeps between bucket sorts (0: the checkpoint interval)
+    int E = 1, nslot = 0;              // sorts per checkpoint segment, sort table slots (ncp x E)
     int P = 0, nreal = 0;
     int mode = DS_KERNELS_TILED;       // requested
     int ppt = 2;                       // rows per wavefront of the tiled scatter kernels (chunk = 4 * ppt rows)
@@ -116,6 +118,8 @@ struct ds_ctx {
     double* d_seg_mj = nullptr;   // K x 10 x P: J and its SVD backward factor per step of a segment
     float* d_gadj = nullptr;      // 18 x P: gathered quantities of the adjoint P2G
     double* d_thpart = nullptr;   // maxgch x 4: per-chunk parameter-gradient partials of a segment
+    float* d_seg_tmp = nullptr;   // a step's state before the permutation of a mid-segment sort (E > 1)
+    float* d_seg_tmp_b = nullptr; // the same for the second segment buffer set
     // second segment buffer set and the recompute's own scratch, for the overlapped adjoint
     float* d_seg_b = nullptr;
     float4* d_seg_grid_b = nullptr;
@@ -347,6 +351,7 @@ void free_overlap(ds_ctx* c)
     dfree(c->d_seg_mj_b);
     dfree(c->d_staging_r);
     dfree(c->d_fbflag_r);
+    dfree(c->d_seg_tmp_b);
     c->overlap_alloc = false;
 }
 
@@ -394,6 +399,7 @@ void free_mode(ds_ctx* c)
     dfree(c->d_seg_mj);
     dfree(c->d_gadj);
     dfree(c->d_thpart);
+    dfree(c->d_seg_tmp);
     dfree(c->d_brows);
     dfree(c->d_rstart);
     dfree(c->d_src);
@@ -459,6 +465,8 @@ int alloc_mode(ds_ctx* c)
     } else {
         c->NB = (c->desc.n_grid + TB - 1) / TB;
         c->NBK = c->B * 3 * c->NB * c->NB * c->NB;
+        c->E = c->K / (c->sort_every > 0 ? c->sort_every : c->K);
+        c->nslot = c->ncp * c->E;
         // chunks of a bucket: ceil(rows / (WAVES * PPT)) <= rows / WAVES + 1, and rows <= particles
         c->maxch = (long)c->P / WAVES + std::min<long>(c->P, c->NBK) + 1;
         c->maxgch = (long)c->P / WG + std::min<long>(c->P, c->NBK) + 1;
@@ -478,15 +486,15 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_cnt, (size_t)c->NBK * 64);
         MALLOC(c->d_celloff, (size_t)c->NBK * 64);
         MALLOC(c->d_bcount, c->NBK);
-        MALLOC(c->d_bnch, (size_t)c->NBK * c->ncp);
+        MALLOC(c->d_bnch, (size_t)c->NBK * c->nslot);
         MALLOC(c->d_bstart, c->NBK);
-        MALLOC(c->d_cstart, (size_t)c->NBK * c->ncp);
+        MALLOC(c->d_cstart, (size_t)c->NBK * c->nslot);
         MALLOC(c->d_tot, 4);
         MALLOC(c->d_tsum, (size_t)4 * ((c->NBK + SCAN_TILE - 1) / SCAN_TILE));
-        MALLOC(c->d_bgch, (size_t)c->NBK * c->ncp);
-        MALLOC(c->d_gstart, (size_t)c->NBK * c->ncp);
-        MALLOC(c->d_gchunks, (size_t)c->maxgch * c->ncp);
-        MALLOC(c->d_cov, (size_t)c->B * c->NB * c->NB * c->NB * c->ncp);
+        MALLOC(c->d_bgch, (size_t)c->NBK * c->nslot);
+        MALLOC(c->d_gstart, (size_t)c->NBK * c->nslot);
+        MALLOC(c->d_gchunks, (size_t)c->maxgch * c->nslot);
+        MALLOC(c->d_cov, (size_t)c->B * c->NB * c->NB * c->NB * c->nslot);
         MALLOC(c->d_fbflag, (size_t)c->B * c->NB * c->NB * c->NB);
         MALLOC(c->d_j, c->P);
         MALLOC(c->d_seg_mj, (size_t)10 * c->P * c->K);
@@ -494,15 +502,16 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_thpart, (size_t)4 * c->maxgch);
         MALLOC(c->d_brows, c->NBK);
         MALLOC(c->d_rstart, c->NBK);
-        MALLOC(c->d_rows, (size_t)(c->P + 1) * c->ncp);
+        MALLOC(c->d_rows, (size_t)(c->P + 1) * c->nslot);
         MALLOC(c->d_key, c->P);
         MALLOC(c->d_rank, c->P);
         MALLOC(c->d_rrpos, c->P);
         MALLOC(c->d_clist, c->P);
-        MALLOC(c->d_chunks, (size_t)c->maxch * c->ncp);
-        MALLOC(c->d_src, (size_t)c->P * c->ncp);
-        MALLOC(c->d_counts, (size_t)4 * c->ncp);
-        fill_i32(c, c->d_counts, 4L * c->ncp, 0);
+        MALLOC(c->d_chunks, (size_t)c->maxch * c->nslot);
+        MALLOC(c->d_src, (size_t)c->P * c->nslot);
+        MALLOC(c->d_counts, (size_t)4 * c->nslot);
+        fill_i32(c, c->d_counts, 4L * c->nslot, 0);
+        if (c->E > 1) MALLOC(c->d_seg_tmp, SF);
         // staging slots for either chunk set at its largest
         c->staging_cap = std::max(c->maxch, c->maxgch);
         MALLOC(c->d_staging, (size_t)c->staging_cap * TNODES);
@@ -534,6 +543,7 @@ int alloc_overlap(ds_ctx* c)
     MALLOC(c->d_seg_mj_b, (size_t)10 * c->P * c->K);
     MALLOC(c->d_staging_r, (size_t)c->staging_cap * TNODES);
     MALLOC(c->d_fbflag_r, (size_t)c->B * c->NB * c->NB * c->NB);
+    if (c->E > 1) MALLOC(c->d_seg_tmp_b, SF);
 #undef MALLOC
     fill_i32(c, c->d_fbflag_r, (long)c->B * c->NB * c->NB * c->NB, 0);
     if (!c->stream2) HIPCHK(c, hipStreamCreate(&c->stream2));
@@ -613,10 +623,11 @@ int backward_ref(ds_ctx* c)
 
 // Sort the current state into checkpoint slot cp (bucket order, round-robin within buckets),
 // build that checkpoint's chunk table and record where every slot came from.
-int sort_into_checkpoint(ds_ctx* c, int cp)
+// Bucket sort of the current state (d_state) into dst, with the chunk tables, row table,
+// reached-block flags and new-to-old slot map in table slot `slot`.
+int sort_into(ds_ctx* c, int slot, float* dst)
 {
-    const long SF = c->stateFloats();
-    float* dst = c->d_ckpt + (size_t)cp * SF;
+    const int cp = slot;   // table slot (checkpoint x sorts per segment)
     Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
     int* src = c->d_src + (size_t)cp * c->P;
     int* bnch = c->d_bnch + (size_t)cp * c->NBK;
@@ -770,24 +781,34 @@ void liquid_j_tiled(ds_ctx* c, const Exec& x, const float* st, int cp, double* m
                          ngc, (int)c->maxgch, c->d_j, mj, c->dsc, c->P);
 }
 
+// Substeps between sorts and the table slot of step s: the sorts of checkpoint segment cp sit
+// at its start and every R substeps after it, in slots cp * E .. cp * E + E - 1.
+inline int sort_interval(const ds_ctx* c) { return c->K / c->E; }
+inline int table_slot(const ds_ctx* c, int s) { return (s / c->K) * c->E + (s % c->K) / sort_interval(c); }
+
 int forward_tiled(ds_ctx* c, int nsteps)
 {
     const long SF = c->stateFloats();
+    const int R = sort_interval(c);
     for (int i = 0; i < nsteps; i++) {
         const int s = c->step;
         const int cp = s / c->K;
+        const int slot = table_slot(c, s);
         const float* in = c->d_state;
         c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
-        if (s % c->K == 0) {
+        if (s % R == 0) {
+            // a checkpoint sort keeps its sorted state; a sort inside a segment goes through the
+            // segment buffer, which nothing else uses during a forward rollout
+            float* dst = s % c->K == 0 ? c->d_ckpt + (size_t)cp * SF : c->d_seg;
             {
                 KernelTimer kt(c, DS_K_SORT);
-                RCHK(sort_into_checkpoint(c, cp));
+                RCHK(sort_into(c, slot, dst));
             }
-            in = c->d_ckpt + (size_t)cp * SF;
-            liquid_j_tiled(c, main_exec(c), in, cp);
+            in = dst;
+            liquid_j_tiled(c, main_exec(c), in, slot);
         }
-        p2g_reduce_tiled(c, main_exec(c), in, c->d_state, cp, nullptr, nullptr);
-        g2p_tiled(c, main_exec(c), in, c->d_state, cp, s, true);
+        p2g_reduce_tiled(c, main_exec(c), in, c->d_state, slot, nullptr, nullptr);
+        g2p_tiled(c, main_exec(c), in, c->d_state, slot, s, true);
         c->step++;
     }
     return DS_OK;
@@ -800,11 +821,12 @@ struct SegSet {
     float4* grid;
     float* aff;
     double* mj;
+    float* tmp;   // a step's state before a mid-segment sort's permutation (sorts per segment > 1)
 };
 inline SegSet seg_set(const ds_ctx* c, int i)
 {
-    return i == 0 ? SegSet{c->d_seg, c->d_seg_grid, c->d_seg_aff, c->d_seg_mj}
-                  : SegSet{c->d_seg_b, c->d_seg_grid_b, c->d_seg_aff_b, c->d_seg_mj_b};
+    return i == 0 ? SegSet{c->d_seg, c->d_seg_grid, c->d_seg_aff, c->d_seg_mj, c->d_seg_tmp}
+                  : SegSet{c->d_seg_b, c->d_seg_grid_b, c->d_seg_aff_b, c->d_seg_mj_b, c->d_seg_tmp_b};
 }
 
 // Substeps [s0, s1) of checkpoint segment cp (s1 = min(s0 + K, recorded steps)).
@@ -823,14 +845,29 @@ void recompute_segment(ds_ctx* c, const Exec& x, int cp, int S, const SegSet& z)
     int s0, L;
     segment_range(c, cp, S, s0, L);
     const float* ck = c->d_ckpt + (size_t)cp * SF;
+    const int R = sort_interval(c);
     auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : z.seg + (size_t)k * SF; };
-    liquid_j_tiled(c, x, ck, cp, z.mj);
+    auto mj_of = [&](int k) -> double* { return z.mj + (size_t)k * 10 * c->P; };
+    liquid_j_tiled(c, x, ck, table_slot(c, s0), z.mj);
     for (int k = 0; k < L; k++) {
         const int s = s0 + k;
+        const int slot = table_slot(c, s);
         c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
-        p2g_reduce_tiled(c, x, st_of(k), k + 1 < L ? st_of(k + 1) : nullptr, cp, z.grid + (size_t)k * G,
-                         z.aff + (size_t)k * 9 * c->P);
-        if (k + 1 < L) g2p_tiled(c, x, st_of(k), st_of(k + 1), cp, s, false, z.mj + (size_t)(k + 1) * 10 * c->P);
+        // the forward sorted again before step k + 1: this step writes the old order into tmp,
+        // which the forward's permutation of that sort takes to the step's buffer
+        const bool resort = k + 1 < L && (k + 1) % R == 0;
+        float* next = k + 1 < L ? (resort ? z.tmp : st_of(k + 1)) : nullptr;
+        p2g_reduce_tiled(c, x, st_of(k), next, slot, z.grid + (size_t)k * G, z.aff + (size_t)k * 9 * c->P);
+        if (k + 1 >= L) continue;
+        if (!resort) {
+            g2p_tiled(c, x, st_of(k), next, slot, s, false, mj_of(k + 1));
+            continue;
+        }
+        g2p_tiled(c, x, st_of(k), next, slot, s, false);
+        const int slot2 = table_slot(c, s + 1);
+        hipLaunchKernelGGL(k_permute_state, dim3(nblocks(c->nreal)), dim3(256), 0, x.stream, z.tmp, st_of(k + 1),
+                           c->d_src + (size_t)slot2 * c->P, c->nreal, c->P);
+        liquid_j_tiled(c, x, st_of(k + 1), slot2, mj_of(k + 1));
     }
 }
 
@@ -844,14 +881,9 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
     const int nb3 = c->NB * c->NB * c->NB;
     const dim3 rgrid = chunk_grid(c, c->maxch), ggrid = chunk_grid(c, c->maxgch);
     const int maxch = (int)c->maxch, maxgch = (int)c->maxgch;
+    const int R = sort_interval(c);
     int s0, L;
     segment_range(c, cp, S, s0, L);
-    const int* counts = c->d_counts + 4 * cp;   // particles, row chunks, rows, gather chunks
-    const Chunk* chunks = c->d_chunks + (size_t)cp * c->maxch;
-    const int* rows = c->d_rows + (size_t)cp * (c->P + 1);
-    const GChunk* gchunks = c->d_gchunks + (size_t)cp * c->maxgch;
-    const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
-    const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
     const float* ck = c->d_ckpt + (size_t)cp * SF;
     auto st_of = [&](int k) -> float* { return k == 0 ? (float*)ck : z.seg + (size_t)k * SF; };
     auto grid_of = [&](int k) -> float4* { return z.grid + (size_t)k * G; };
@@ -860,6 +892,14 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
     fill_zero_f64(c, c->d_thpart, 4L * c->maxgch);
     for (int k = L - 1; k >= 0; k--) {
         const int s = s0 + k;
+        // the tables of the sort this step ran under
+        const int slot = table_slot(c, s);
+        const int* counts = c->d_counts + 4 * slot;   // particles, row chunks, rows, gather chunks
+        const Chunk* chunks = c->d_chunks + (size_t)slot * c->maxch;
+        const int* rows = c->d_rows + (size_t)slot * (c->P + 1);
+        const GChunk* gchunks = c->d_gchunks + (size_t)slot * c->maxgch;
+        const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)slot * c->NBK;
+        const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)slot * c->NBK;
         c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
         {
             KernelTimer kt(c, DS_K_G2P_ADJ, true);
@@ -875,7 +915,7 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
         {
             KernelTimer kt(c, DS_K_VBAR_REDUCE, true);
             DS_LAUNCH_T(kt, k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
-                        c->d_cov + (size_t)cp * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2, c->dsc,
+                        c->d_cov + (size_t)slot * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2, c->dsc,
                         c->NB);
         }
         if (c->adj_split) {
@@ -895,15 +935,18 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
                                aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2, c->d_gadj,
                                c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
         }
-    }
-    hipLaunchKernelGGL(k_thpart_reduce, dim3((unsigned)std::min<long>((c->maxgch + 255) / 256, 1024)), dim3(256), 0,
-                       c->stream, gchunks, counts + 3, maxgch, c->d_thpart, c->d_thbar);
-    if (cp > 0) {
-        // the adjoint in the previous checkpoint's particle order, back into the fixed buffer
-        hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
-                           c->d_src + (size_t)cp * c->P, c->nreal, c->P);
-        hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)std::min<long>(nblocks(SF), 8192)), dim3(256), 0, c->stream,
-                           c->d_adj, c->d_adj2, SF);
+        if (k % R != 0) continue;
+        // first step after a sort: the sort's parameter-gradient partials, and the adjoint back
+        // into the particle order before that sort (none before the very first one)
+        hipLaunchKernelGGL(k_thpart_reduce, dim3((unsigned)std::min<long>((c->maxgch + 255) / 256, 1024)), dim3(256),
+                           0, c->stream, gchunks, counts + 3, maxgch, c->d_thpart, c->d_thbar);
+        if (k > 0) fill_zero_f64(c, c->d_thpart, 4L * c->maxgch);
+        if (k > 0 || cp > 0) {
+            hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,
+                               c->d_src + (size_t)slot * c->P, c->nreal, c->P);
+            hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)std::min<long>(nblocks(SF), 8192)), dim3(256), 0,
+                               c->stream, c->d_adj, c->d_adj2, SF);
+        }
     }
 }
 
@@ -1129,6 +1172,19 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         const long g = value == 0 ? CHUNK_GRID : value;
         if (g != c->chunk_grid) drop_graphs(c);
         c->chunk_grid = g;
+        return DS_OK;
+    }
+    if (option == DS_OPT_SORT_EVERY) {
+        if (value < 0 || value > c->K || (value > 0 && c->K % value != 0))
+            return fail(c, DS_EINVAL, "DS_OPT_SORT_EVERY takes 0 (the checkpoint interval) or a divisor of it");
+        const int v = value == c->K ? 0 : value;
+        if (v != c->sort_every) {
+            // the sort tables are sized by the sorts per segment: reallocated at the next reset
+            if (c->stream) (void)hipStreamSynchronize(c->stream);
+            free_mode(c);
+            c->reset_done = false;
+        }
+        c->sort_every = v;
         return DS_OK;
     }
     if (option == DS_OPT_OVERLAP) {

@@ -1262,6 +1262,20 @@ __global__ void __launch_bounds__(256) k_fill_i32(int* __restrict__ p, long n, i
 
 // Adjoint carried across a re-sort: slot q of the later order came from slot src[q] of the
 // earlier order.
+// The state of a recomputed step into the order a sort of the forward gave it at that step:
+// slot q of out takes slot src[q] of in (real particles only; padding slots keep stale values
+// that no chunk reads).
+__global__ void __launch_bounds__(256) k_permute_state(const float* __restrict__ in, float* __restrict__ out,
+                                                       const int* __restrict__ src, int nreal, int P)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nreal) return;
+    const int p = src[q];
+    const long LP = P;
+#pragma unroll
+    for (int a = 0; a < NSLOT; a++) out[a * LP + q] = in[a * LP + p];
+}
+
 __global__ void __launch_bounds__(256) k_unsort_adj(const float* __restrict__ in, float* __restrict__ out,
                                                     const int* __restrict__ src, int nreal, int P)
 {

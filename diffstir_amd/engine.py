@@ -46,7 +46,7 @@ class Engine:
 
     # execution options of the tiled kernels (ds_set_option ids)
     OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
-    OPT_OVERLAP, OPT_CHUNK_GRID = 8, 9
+    OPT_OVERLAP, OPT_CHUNK_GRID, OPT_SORT_EVERY = 8, 9, 10
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
@@ -83,6 +83,7 @@ class Engine:
         self.configure(particles_per_thread=particles_per_thread,
                        gather_particles_per_thread=gather_particles_per_thread, adjoint_p2g=adjoint_p2g,
                        scatter_chunks=scatter_chunks, overlap=overlap)
+        self.sort_every = 0
         self.set_graph(graph)
         for b, s in enumerate(scenes):
             self.upload_scene(b, s)
@@ -104,12 +105,14 @@ class Engine:
 
     def configure(self, particles_per_thread: Optional[int] = None, gather_particles_per_thread: Optional[int] = None,
                   adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None,
-                  overlap: Optional[bool] = None, chunk_grid: Optional[int] = None):
+                  overlap: Optional[bool] = None, chunk_grid: Optional[int] = None,
+                  sort_every: Optional[int] = None):
         """Execution options of the tiled kernels (they change how the substep runs, never what it
         computes); particle-per-thread changes take effect at the next reset.  `overlap`: the
         adjoint recomputes the next checkpoint segment on a second stream while the current one's
         adjoint runs (a second set of segment buffers).  `chunk_grid`: workgroups per chunk-kernel
-        launch (0: the library default)."""
+        launch (0: the library default).  `sort_every`: substeps between bucket sorts (0: the
+        checkpoint interval; else a divisor of it), from the next reset."""
         if particles_per_thread is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_PPT, int(particles_per_thread)))
             self.particles_per_thread = int(particles_per_thread)
@@ -131,6 +134,9 @@ class Engine:
             self.overlap = bool(overlap)
         if chunk_grid is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_CHUNK_GRID, int(chunk_grid)))
+        if sort_every is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_SORT_EVERY, int(sort_every)))
+            self.sort_every = int(sort_every)
 
     def set_kernel_timing(self, every: int):
         """Sample every N-th substep's kernels with HIP events (0: off)."""
@@ -145,7 +151,7 @@ class Engine:
     def options(self) -> dict:
         return {"particles_per_thread": self.particles_per_thread,
                 "gather_particles_per_thread": self.gather_particles_per_thread, "adjoint_p2g": self.adjoint_p2g,
-                "scatter_chunks": self.scatter_chunks, "overlap": self.overlap}
+                "scatter_chunks": self.scatter_chunks, "overlap": self.overlap, "sort_every": self.sort_every}
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):

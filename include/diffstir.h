@@ -58,6 +58,10 @@ enum {
     DS_OPT_CHUNK_GRID = 9,             /* workgroups of each tiled chunk kernel launch (the chunk count lives
                                           on the device; chunks beyond the grid loop): 0 = 16384 (default),
                                           or 64..1048576 */
+    DS_OPT_SORT_EVERY = 10,            /* substeps between bucket sorts: 0 (default) = the checkpoint interval,
+                                          or a divisor of it.  Sorting more often keeps drifting particles
+                                          inside their chunk's tile; the adjoint replays the forward's sorts.
+                                          Reallocates the sort tables: takes effect at the next ds_reset */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or

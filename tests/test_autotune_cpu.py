@@ -52,19 +52,30 @@ def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
     eng = _FakeEngine()
     f = tmp_path / "tune.json"
     res = A.tune(scenes, ckpt_every=5, engine=eng, timer=timer, cache_file=str(f), warm_steps=0)
-    assert res["options"] == want
-    assert eng.configured[-1] == want
-    assert len(calls) == len(A.candidates()) and all(s == 10 for _, s in calls)   # 2 checkpoint segments
+    want2 = dict(want, sort_every=0)   # the second stage keeps sorting once per interval
+    assert res["options"] == want2
+    assert eng.configured[-1] == want2
+    n2 = len(A.sort_candidates(5)) - 1
+    assert len(calls) == len(A.candidates()) + n2 and all(s == 10 for _, s in calls)   # 2 checkpoint segments
+    assert all(c == dict(want, sort_every=R) for (c, _), R in zip(calls[-n2:], A.sort_candidates(5)[1:]))
     # cached in memory and on disk by signature
     again = A.tune(scenes, ckpt_every=5, engine=eng, timer=lambda *a: pytest.fail("re-timed"))
-    assert again["options"] == want
+    assert again["options"] == want2
     disk = json.loads(f.read_text())
-    assert disk[res["signature"]]["options"] == want
+    assert disk[res["signature"]]["options"] == want2
     A.clear_cache()
     from_disk = A.tune(scenes, ckpt_every=5, engine=eng, timer=lambda *a: pytest.fail("re-timed"),
                        cache_file=str(f))
-    assert from_disk["options"] == want
+    assert from_disk["options"] == want2
     A.clear_cache()
+
+
+def test_sort_candidates_divide_the_interval():
+    assert A.sort_candidates(100) == [0, 50, 25]
+    assert A.sort_candidates(50) == [0, 25, 10]
+    assert A.sort_candidates(5) == [0]
+    for K in (5, 6, 7, 12, 50, 100):
+        assert all(K % R == 0 for R in A.sort_candidates(K)[1:])
 
 
 def test_signature_separates_workloads():

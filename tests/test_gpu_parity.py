@@ -242,10 +242,12 @@ def test_autotune_picks_a_candidate_on_the_device():
     A.clear_cache()
     sc = S.tiny_scene(16, 30, seed=4)
     print("[gpu] autotune", flush=True)
-    res = A.tune([sc], ckpt_every=5, substeps=10, repeats=1)
-    assert res["options"] in A.candidates()
-    assert res["warm_steps"] == (sc.max_steps - 1) // 2
-    assert len(res["timings_ms"]) == len(A.candidates())
+    res = A.tune([sc], ckpt_every=6, substeps=12, repeats=1)
+    opts = dict(res["options"])
+    assert opts.pop("sort_every") in A.sort_candidates(6)
+    assert opts in A.candidates()
+    assert res["warm_steps"] == min((sc.max_steps - 1) // 2, sc.max_steps - 1 - 12)
+    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1
     assert all(np.isfinite(t["ms"]) and t["ms"] > 0 for t in res["timings_ms"])
     A.clear_cache()
 
@@ -330,3 +332,62 @@ def test_svd3_backward_matches_torch_autograd():
     (gF_t,) = torch.autograd.grad(L, Ft)
     gF = svd3_backward(U.detach().numpy(), Sg.detach().numpy(), V.detach().numpy(), gU, gS, gV)
     np.testing.assert_allclose(gF, gF_t.numpy(), rtol=1e-8, atol=1e-10)
+
+
+def test_particle_order_does_not_matter():
+    """Shuffling the liquid particles of a scene changes only the FP32 summation order inside the
+    tiles (the sort ranks each cell's particles by their given index): forces, gradient and the
+    final state (read back in the given order) agree to that rounding."""
+    sc = S.tiny_scene(16, 30, seed=7)
+    perm = np.random.default_rng(3).permutation(sc.n_liquid)
+    shuffled = dataclasses.replace(sc, x_liquid=np.ascontiguousarray(sc.x_liquid[perm]))
+    out = []
+    for scene in (sc, shuffled):
+        print(f"[gpu] particle order ({'shuffled' if scene is shuffled else 'given'})", flush=True)
+        with Engine([scene], ckpt_every=8) as e:
+            e.reset()
+            e.forward()
+            F = e.forces()[0]
+            st = e.read_liquid(0)
+            _, g = compute_loss(F, 0.5 * F, 1)
+            out.append((F, st, e.backward(g[None])[0]))
+    (Fa, sa, da), (Fb, sb, db) = out
+    assert force_err(Fb, Fa) <= 1e-5
+    inv = np.argsort(perm)
+    for k in ("x", "v", "C", "F"):
+        np.testing.assert_allclose(sb[k][inv], sa[k], rtol=1e-4, atol=1e-5 * np.abs(sa[k]).max())
+    assert grad_err(db, da) <= 1e-4
+
+
+def test_sorting_more_often_gives_the_same_result():
+    """Bucket sorts every 30 or 12 substeps inside a 60-substep checkpoint interval (the adjoint
+    replays the forward's sorts: it permutes the recomputed state with the forward's maps and
+    un-permutes the particle adjoint at each sort) give the once-per-interval forces, gradient and
+    state to FP32 summation-order noise: launched in several forward calls, replayed as a hipGraph,
+    with and without the overlapped recompute."""
+    sc = S.stirring_scene(32, 150, fill_radius=0.25, fill_height=0.2, stick_radius=0.05, path_rx=0.15, path_rz=0.15,
+                          omega=2 * np.pi * 3, seed=2)
+    res = {}
+    for R, graph, overlap in ((0, False, True), (30, False, True), (12, False, False), (12, True, True)):
+        print(f"[gpu] sort every {R or 60}, graph {graph}, overlap {overlap}", flush=True)
+        with Engine([sc], ckpt_every=60, graph=graph, overlap=overlap) as e:
+            e.configure(sort_every=R)
+            for rep in range(2):
+                e.reset()
+                if graph:
+                    e.forward()
+                else:
+                    e.forward(7)
+                    e.forward(50)
+                    e.forward()
+                F = e.forces()[0]
+                st = e.read_liquid(0)
+                _, g = compute_loss(F, 0.5 * F, 1)
+                d = e.backward(g[None])[0]
+            res[(R, graph, overlap)] = (F, st, d)
+    Fa, sa, da = res[(0, False, True)]
+    for key, (Fb, sb, db) in res.items():
+        assert force_err(Fb, Fa) <= 1e-5, key
+        for k in ("x", "v", "C", "F"):
+            np.testing.assert_allclose(sb[k], sa[k], rtol=1e-4, atol=1e-5 * np.abs(sa[k]).max(), err_msg=str(key))
+        assert grad_err(db, da) <= 1e-4, key
