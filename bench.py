@@ -93,7 +93,7 @@ def device_free_bytes() -> int:
 
 def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10,
             autotune: bool = True, graph: bool = True, kernel_sample: bool = True, substeps: int = 0,
-            overlap: bool = True, sort_every: Optional[int] = None):
+            overlap: bool = True, sort_every: Optional[int] = None, reduce_parts: Optional[int] = None):
     scenes = S.workload(workload)
     if substeps:
         # a shortened rollout (profiling runs whose every launch is serialised by the profiler)
@@ -113,6 +113,8 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
         log(f"[bench] autotuned in {time.perf_counter() - t0:.1f} s: {tuned['options']}")
     if sort_every is not None:
         eng.configure(sort_every=sort_every)
+    if reduce_parts is not None:
+        eng.configure(reduce_parts=reduce_parts)
     graph = graph and kernels == "tiled"
     eng.set_graph(graph)
     targets = make_targets(eng, scenes)
@@ -289,6 +291,8 @@ def main():
     ap.add_argument("--sort-every", type=int, default=None,
                     help="sort the particles every this many substeps (a divisor of --ckpt; 0 = once per "
                          "checkpoint interval); overrides the autotuner's choice")
+    ap.add_argument("--reduce-parts", type=int, default=None, choices=[1, 2, 4],
+                    help="wavefronts per node block of the grid reductions; overrides the autotuner's choice")
     a = ap.parse_args()
     if a.calibrate_copy_mib:
         calibrate_copy(a.calibrate_copy_mib)
@@ -298,7 +302,7 @@ def main():
               "and nothing here uses a second GPU", file=sys.stderr)
         sys.exit(2)
     r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph,
-                substeps=a.substeps, overlap=not a.no_overlap, sort_every=a.sort_every)
+                substeps=a.substeps, overlap=not a.no_overlap, sort_every=a.sort_every, reduce_parts=a.reduce_parts)
     value = r["psteps"] / r["elapsed"]
     ms_per_step = 1000.0 * r["elapsed"] / a.steps
     bpps = fwd_adj_bytes(a.ckpt)

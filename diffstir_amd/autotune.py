@@ -13,9 +13,14 @@ The options change how a substep runs on the device, never what it computes:
   ("packed": full wavefronts, lanes sharing a base served in successive passes).  With packed
   chunks the rows-per-wavefront choice does not apply.
 
-A second stage then tries sorting the particles into buckets more often than once per checkpoint
-interval (``sort_every``: half or a quarter/fifth of it): where particles drift far within an
-interval (the n_grid 128 workload) it keeps their stencils inside their chunk's tile.
+Later stages then refine the chosen set one option at a time:
+
+* ``sort_every`` -- sorting the particles into buckets more often than once per checkpoint
+  interval (half or a quarter/fifth of it): where particles drift far within an interval (the
+  n_grid 128 workload) it keeps their stencils inside their chunk's tile;
+* ``reduce_parts`` -- 1, 2 or 4 wavefronts per node block of the grid reductions: where buckets
+  hold many chunks (one clumped instance) the rounds of slot loads split over the wavefronts;
+  where they hold one or two (large batches) the extra wavefronts only take up slots.
 
 Which choice is fastest depends on the workload's size and on how the particles cluster, so each
 candidate is timed on the workload itself: a short checkpointed forward + adjoint rollout
@@ -43,6 +48,9 @@ SPACE = {
 DEFAULT = {"particles_per_thread": 2, "gather_particles_per_thread": 1, "adjoint_p2g": "fused",
            "scatter_chunks": "rows"}
 
+# options refined after the first stage, in order, with their default (first) values
+LATER_STAGES = ("sort_every", "reduce_parts")
+
 _CACHE: Dict[str, dict] = {}
 
 
@@ -68,6 +76,19 @@ def sort_candidates(ckpt_every: int) -> List[int]:
     divisors K/2, K/4, K/5."""
     K = int(ckpt_every)
     return [0] + [K // d for d in (2, 4, 5) if K % d == 0 and K // d >= 2][:2]
+
+
+def stage_candidates(key: str, ckpt_every: int) -> List:
+    """Values of a later-stage option, its default first."""
+    if key == "sort_every":
+        return sort_candidates(ckpt_every)
+    if key == "reduce_parts":
+        return [1, 2, 4]
+    raise KeyError(key)
+
+
+def later_defaults(ckpt_every: int) -> dict:
+    return {k: stage_candidates(k, ckpt_every)[0] for k in LATER_STAGES}
 
 
 def signature(scenes, ckpt_every: int) -> str:
@@ -108,7 +129,7 @@ def warmed_scenes(scenes, engine, warm_steps: int):
     options) reaches after `warm_steps` substeps: liquid and stick positions of that step, the
     stick trajectory from that frame on."""
     import dataclasses
-    engine.configure(**DEFAULT)
+    engine.configure(**DEFAULT, sort_every=0, reduce_parts=1)
     engine.reset()
     engine.forward(warm_steps)
     out = []
@@ -153,22 +174,25 @@ def tune(scenes, ckpt_every: int = 50, substeps: Optional[int] = None, space: Op
             for b, s in enumerate(warmed_scenes(scenes, eng, warm_steps)):
                 eng.upload_scene(b, s)
         timings = []
-        for cand in candidates(space):
+
+        def run(cand):
             t = timer(eng, cand, substeps, repeats)
             timings.append((cand, t))
             if verbose:
                 print(f"[autotune] {cand}: {t:.2f} ms", flush=True)
-        best = pick(timings)
-        # second stage: the sort interval, with the chosen options
-        stage2 = [(dict(best, sort_every=0), min(t for c, t in timings if c == best))]
-        for R in sort_candidates(ckpt_every)[1:]:
-            cand = dict(best, sort_every=R)
-            t = timer(eng, cand, substeps, repeats)
-            stage2.append((cand, t))
-            timings.append((cand, t))
-            if verbose:
-                print(f"[autotune] {cand}: {t:.2f} ms", flush=True)
-        best = pick(stage2)
+            return t
+
+        # first stage: the kernel options, with the later stages' options at their defaults
+        first = [dict(c, **later_defaults(ckpt_every)) for c in candidates(space)]
+        best = pick((c, run(c)) for c in first)
+        best_t = min(t for c, t in timings if c == best)
+        # later stages: one option at a time, with the options chosen so far
+        for key in LATER_STAGES:
+            vals = stage_candidates(key, ckpt_every)
+            stage = [(best, best_t)] + [(dict(best, **{key: v}), None) for v in vals if v != best[key]]
+            stage = [(c, t if t is not None else run(c)) for c, t in stage]
+            best = pick(stage)
+            best_t = min(t for c, t in stage if c == best)
         if warm_steps > 0:
             for b, s in enumerate(scenes):
                 eng.upload_scene(b, s)

@@ -46,6 +46,7 @@ struct ds_ctx {
     int gppt = 1;                      // particles per thread of the tiled gather kernels (256 * gppt)
     int gppt_sorted = 1;
     bool adj_split = false;            // adjoint P2G as two kernels (gather, stress) instead of one
+    int rparts = 1;                    // wavefronts per node block of the grid reductions (1, 2, 4)
     bool packed = false;               // scatter kernels on the packed (gather) chunks with claim passes
     bool use_graph = false;            // replay whole-rollout forward / adjoint launch sequences as hipGraphs
     bool overlap = true;               // adjoint: recompute segment k-1 on a second stream during segment k's adjoint
@@ -711,6 +712,13 @@ inline dim3 chunk_grid(const ds_ctx* c, long maxchunks)
         else if ((ppt) == 2) DS_LAUNCH_T(kt, (kern<2, b>), __VA_ARGS__);                         \
         else DS_LAUNCH_T(kt, (kern<4, b>), __VA_ARGS__);                                         \
     } while (0)
+// A grid reduction with `parts` wavefronts per node block (64 * parts threads).
+#define DS_PARTS_DISPATCH_T(kt, parts, kern, grid, stream, ...)                                  \
+    do {                                                                                         \
+        if ((parts) == 1) DS_LAUNCH_T(kt, kern<1>, grid, dim3(64), 0, stream, __VA_ARGS__);      \
+        else if ((parts) == 2) DS_LAUNCH_T(kt, kern<2>, grid, dim3(128), 0, stream, __VA_ARGS__); \
+        else DS_LAUNCH_T(kt, kern<4>, grid, dim3(256), 0, stream, __VA_ARGS__);                  \
+    } while (0)
 
 // Where the forward-type kernels (P2G, grid reduction, G2P, J) run: the stream and the scratch
 // they write besides their outputs.  The forward and a sequential recompute use the main stream's;
@@ -746,8 +754,8 @@ void p2g_reduce_tiled(ds_ctx* c, const Exec& x, const float* in, float* fout, in
     const int nb3 = c->NB * c->NB * c->NB;
     KernelTimer kt(c, DS_K_GRID_REDUCE, true);
     const int* cov = c->d_cov + (size_t)cp * c->B * nb3;
-    DS_LAUNCH_T(kt, k_grid_reduce, dim3(c->B * nb3), dim3(64), 0, x.stream, x.staging, bnch, cstart, cov, c->d_pfb,
-                x.fbflag, pout, c->d_vgrid, c->dsc, c->NB);
+    DS_PARTS_DISPATCH_T(kt, c->rparts, k_grid_reduce, dim3(c->B * nb3), x.stream, x.staging, bnch, cstart, cov,
+                        c->d_pfb, x.fbflag, pout, c->d_vgrid, c->dsc, c->NB);
 }
 
 void g2p_tiled(ds_ctx* c, const Exec& x, const float* in, float* out, int cp, int f, bool record,
@@ -914,9 +922,9 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
         }
         {
             KernelTimer kt(c, DS_K_VBAR_REDUCE, true);
-            DS_LAUNCH_T(kt, k_vbar_reduce, dim3(c->B * nb3), dim3(64), 0, c->stream, c->d_staging, bnch, cstart,
-                        c->d_cov + (size_t)slot * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k), c->d_pbar2, c->dsc,
-                        c->NB);
+            DS_PARTS_DISPATCH_T(kt, c->rparts, k_vbar_reduce, dim3(c->B * nb3), c->stream, c->d_staging, bnch,
+                                cstart, c->d_cov + (size_t)slot * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k),
+                                c->d_pbar2, c->dsc, c->NB);
         }
         if (c->adj_split) {
             {
@@ -1185,6 +1193,12 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
             c->reset_done = false;
         }
         c->sort_every = v;
+        return DS_OK;
+    }
+    if (option == DS_OPT_REDUCE_PARTS) {
+        if (value != 1 && value != 2 && value != 4) return fail(c, DS_EINVAL, "DS_OPT_REDUCE_PARTS takes 1, 2 or 4");
+        if (c->rparts != value) drop_graphs(c);
+        c->rparts = (int)value;
         return DS_OK;
     }
     if (option == DS_OPT_OVERLAP) {

@@ -32,8 +32,10 @@
 //   k_g2p_adj_tiled   adjoint G2P: velocity tile from momentum/mass, dL/dv scatter to staging
 //   k_vbar_reduce     staged dL/dv -> adjoint grid update -> dL/d(momentum, mass)
 //   k_p2g_adj_tiled   adjoint P2G gather, FP64 parameter-gradient reduction
-// LDS tiles are component planes with strides (x 68, y 8, z 1): for the 64 cells of a block
-// the node offsets of one stencil point fall in 32 distinct banks per half-wavefront.
+// The P2G wave tiles hold one float4 (momentum, mass) per node in the staging slot's node order
+// (x 64, y 8, z 1), so a node's read-modify-write is one 16-byte LDS read and one write.  The
+// other LDS tiles are component planes with strides (x 68, y 8, z 1): for the 64 cells of a
+// block the node offsets of one stencil point fall in 32 distinct banks per half-wavefront.
 #include "ds_particle.h"
 
 namespace ds {
@@ -131,18 +133,16 @@ __device__ __forceinline__ void mark_fallback_blocks(int* __restrict__ fbflag, i
 
 // Read-modify-write scatter into a wavefront's private tile (see the claim protocol above).
 // The barrier after each node keeps the next node's reads behind this node's writes: another
-// lane's stencil may reach the same node at a different offset.
+// lane's stencil may reach the same node at a different offset.  WaveScatter4: one float4 per
+// node (TNODES per wave, staging order); WaveScatter3: three component planes.
 struct WaveScatter4 {
-    float* t;
+    float4* t;
     int x0, y0, z0;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c, float d)
     {
-        float* q = t + tile_id(i - x0, j - y0, k - z0);
-        const float qa = q[0], qb = q[TILE], qc = q[2 * TILE], qd = q[3 * TILE];
-        q[0] = qa + a;
-        q[TILE] = qb + b;
-        q[2 * TILE] = qc + c;
-        q[3 * TILE] = qd + d;
+        float4* q = t + ((i - x0) * TS + (j - y0)) * TS + (k - z0);
+        const float4 o = *q;
+        *q = make_float4(o.x + a, o.y + b, o.z + c, o.w + d);
         lds_order();
     }
 };
@@ -208,6 +208,24 @@ __device__ __forceinline__ int node_key(const Stencil& s, int x0, int y0, int z0
     return ((s.b[0] - x0) * TS + (s.b[1] - y0)) * TS + (s.b[2] - z0);
 }
 
+// Sum the float4 wave tiles (TNODES each, staging order) of the workgroup into the chunk's
+// staging slot.
+__device__ __forceinline__ void store_staging4(const float4* wt, float4* __restrict__ slot)
+{
+    for (int e = threadIdx.x; e < TNODES; e += WG) {
+        float4 v = wt[e];
+#pragma unroll
+        for (int w = 1; w < WAVES; w++) {
+            const float4 u = wt[w * TNODES + e];
+            v.x += u.x;
+            v.y += u.y;
+            v.z += u.z;
+            v.w += u.w;
+        }
+        slot[e] = v;
+    }
+}
+
 // Sum the wave tiles (ncomp planes each, stride wstride floats) of the workgroup into the
 // chunk's staging slot: TNODES float4 in (lx, ly, lz) order, unused components 0.
 __device__ __forceinline__ void store_staging(const float* wt, int wstride, int ncomp, float4* __restrict__ slot)
@@ -225,8 +243,13 @@ __device__ __forceinline__ void store_staging(const float* wt, int wstride, int 
 
 // ---------------------------------------------------------------------------- forward
 
+// Row chunks: at most 128 registers (four wavefronts per SIMD, as the LDS allows); the packed
+// form's claim loop needs more.
+template <class CH> struct P2GMinWaves { static constexpr int value = 1; };
+template <> struct P2GMinWaves<Chunk> { static constexpr int value = 4; };
+
 template <int PPT, class CH>
-__global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
+__global__ void __launch_bounds__(WG, P2GMinWaves<CH>::value) k_p2g_tiled(const float* __restrict__ in, float* __restrict__ fout,
                                                   float* __restrict__ aout, const CH* __restrict__ chunks,
                                                   const int* __restrict__ rows,
                                                   const int* __restrict__ nchunks_p, int maxchunks,
@@ -237,7 +260,7 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
     const int nchunks = min(*nchunks_p, maxchunks);
-    __shared__ __align__(16) float wt[WAVES * 4 * TILE];
+    __shared__ float4 wt[WAVES * TNODES];
     __shared__ unsigned char own[WAVES * TNODES];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
@@ -258,9 +281,9 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
             load_particle_t(in, LP, p, liq, q);
             if (liq) J = jin[p];
         }
-        zero_lds(wt, WAVES * 4 * TILE);
+        zero_lds(reinterpret_cast<float*>(wt), WAVES * 4 * TNODES);
         const InstParams ip = load_params(sc, params, ch.inst);
-        WaveScatter4 acc{wt + wave * 4 * TILE, x0, y0, z0};
+        WaveScatter4 acc{wt + wave * TNODES, x0, y0, z0};
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
             Stencil s;
@@ -307,7 +330,7 @@ __global__ void __launch_bounds__(WG) k_p2g_tiled(const float* __restrict__ in, 
             p = pn;
         }
         __syncthreads();
-        store_staging(wt, 4 * TILE, 4, staging + (long)c * TNODES);
+        store_staging4(wt, staging + (long)c * TNODES);
         __syncthreads();
     }
 }
@@ -338,8 +361,10 @@ __device__ __forceinline__ void load_neighbour_buckets(int* s_nc, int* s_cs, con
 // for local nodes 0..2, b and b+1 for local node 3; the node's index in the tile of bucket
 // b + d is local - 4d + 1.  The 24 (species, covering bucket) pairs are visited together:
 // round k loads the k-th chunk slot of every pair at once, so a round costs one memory latency.
+// This call takes rounds k0, k0 + kstep, ... (the parts of a split reduction).
 __device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __restrict__ staging, const int* s_nc,
-                                                   const int* s_cs, int lx, int ly, int lz, int ncomp)
+                                                   const int* s_cs, int lx, int ly, int lz, int ncomp, int k0 = 0,
+                                                   int kstep = 1)
 {
     const int ox = lx < 3 ? -1 : 0, oy = ly < 3 ? -1 : 0, oz = lz < 3 ? -1 : 0;
     int first[24], nc[24];
@@ -353,7 +378,7 @@ __device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __
         first[q] = s_cs[r] * TNODES + e;
         kmax = max(kmax, nc[q]);
     }
-    for (int k = 0; k < kmax; k++) {
+    for (int k = k0; k < kmax; k += kstep) {
         float4 v[24];
 #pragma unroll
         for (int q = 0; q < 24; q++)
@@ -368,19 +393,58 @@ __device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __
     }
 }
 
-// One thread per grid node (a workgroup of 64 = one 4x4x4 node block): the sum of the staging
-// slots of the chunks whose tiles cover the node, in a fixed order, plus the fall-back grid
-// (read and cleared for the next step); then the grid update.  Writes the velocity (vout) and,
-// when pout is given, the momentum/mass.
+// The staged sum of one node of a node block in NPART parts: wavefront w of the workgroup takes
+// rounds w, w + NPART, ... of the covering slots (so a block whose buckets hold many chunks
+// waits for ceil(rounds / NPART) memory latencies instead of all of them) and part 0 adds the
+// others' partial sums in wavefront order.  Part 0 also takes the fall-back grid (read and
+// cleared) first.  Every thread of the workgroup must call this; the result is meaningful in
+// part 0 only.
+template <int NPART>
+__device__ __forceinline__ float4 staged_node_sum(const float4* __restrict__ staging, const int* s_nc, const int* s_cs,
+                                                  float4* __restrict__ fallback, bool fl, long gid, bool inb, int lx,
+                                                  int ly, int lz, int ncomp)
+{
+    __shared__ float4 s_part[NPART > 1 ? (NPART - 1) * 64 : 1];
+    const int node = threadIdx.x & 63, part = threadIdx.x >> 6;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (inb) {
+        if (fl && part == 0) {
+            acc = fallback[gid];
+            fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, ncomp, part, NPART);
+    }
+    if (NPART > 1) {
+        if (part > 0) s_part[(part - 1) * 64 + node] = acc;
+        __syncthreads();
+        if (part == 0) {
+#pragma unroll
+            for (int w = 0; w < NPART - 1; w++) {
+                const float4 t = s_part[w * 64 + node];
+                acc.x += t.x;
+                acc.y += t.y;
+                acc.z += t.z;
+                acc.w += t.w;
+            }
+        }
+    }
+    return acc;
+}
+
+// One thread per grid node (NPART wavefronts per 4x4x4 node block, see staged_node_sum): the
+// sum of the staging slots of the chunks whose tiles cover the node, in a fixed order, plus the
+// fall-back grid (read and cleared for the next step); then the grid update.  Writes the
+// velocity (vout) and, when pout is given, the momentum/mass.
 // Only blocks that a chunk tile reaches (cov, fixed for a checkpoint segment) or that a
 // fall-back scatter marked this step (fbflag, cleared here) are visited.  The others keep stale
 // values that no kernel reads: gathers read nodes inside a chunk's tile or inside the stencil of
 // a particle that scattered into them in the same step.
-__global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ staging, const int* __restrict__ bnch,
-                                                    const int* __restrict__ cstart, const int* __restrict__ cov,
-                                                    float4* __restrict__ fallback, int* __restrict__ fbflag,
-                                                    float4* __restrict__ pout, float4* __restrict__ vout, DevScene sc,
-                                                    int NB)
+template <int NPART>
+__global__ void __launch_bounds__(64 * NPART) k_grid_reduce(const float4* __restrict__ staging,
+                                                            const int* __restrict__ bnch, const int* __restrict__ cstart,
+                                                            const int* __restrict__ cov, float4* __restrict__ fallback,
+                                                            int* __restrict__ fbflag, float4* __restrict__ pout,
+                                                            float4* __restrict__ vout, DevScene sc, int NB)
 {
     const int fl = fbflag[blockIdx.x];
     if (!cov[blockIdx.x] && !fl) return;
@@ -389,20 +453,17 @@ __global__ void __launch_bounds__(64) k_grid_reduce(const float4* __restrict__ s
     const int nb3 = NB * NB * NB;
     const int inst = blockIdx.x / nb3, blk = blockIdx.x % nb3;
     const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
-    const int lx = threadIdx.x >> 4, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x & 3;
+    const int node = threadIdx.x & 63;
+    const int lx = node >> 4, ly = (node >> 2) & 3, lz = node & 3;
     const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
     __shared__ int s_nc[3 * 27], s_cs[3 * 27];
     load_neighbour_buckets(s_nc, s_cs, bnch, cstart, inst, bx, by, bz, NB);
     __syncthreads();
-    if (gx >= n || gy >= n || gz >= n) return;
-    const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
-    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (fl) {
-        acc = fallback[gid];
-        fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (threadIdx.x == 0) fbflag[blockIdx.x] = 0;
-    }
-    sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, 4);
+    const bool inb = gx < n && gy < n && gz < n;
+    const long gid = (long)inst * nn + (inb ? node_index(n, gx, gy, gz) : 0);
+    if (fl && threadIdx.x == 0) fbflag[blockIdx.x] = 0;
+    const float4 acc = staged_node_sum<NPART>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 4);
+    if (!inb || threadIdx.x >= 64) return;
     if (pout) pout[gid] = acc;
     float v[3];
     NodeRec r;
@@ -674,11 +735,12 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
 // One thread per grid node: dL/dv at the node (staging slots of the covering chunks plus the
 // fall-back grid, cleared for the next step), then the adjoint of the grid update with the
 // node's momentum/mass -> dL/d(momentum, mass) into pbar.
-__global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ staging, const int* __restrict__ bnch,
-                                                    const int* __restrict__ cstart, const int* __restrict__ cov,
-                                                    float4* __restrict__ fallback, int* __restrict__ fbflag,
-                                                    const float4* __restrict__ grid, float4* __restrict__ pbar,
-                                                    DevScene sc, int NB)
+template <int NPART>
+__global__ void __launch_bounds__(64 * NPART, 3) k_vbar_reduce(const float4* __restrict__ staging,
+                                                            const int* __restrict__ bnch, const int* __restrict__ cstart,
+                                                            const int* __restrict__ cov, float4* __restrict__ fallback,
+                                                            int* __restrict__ fbflag, const float4* __restrict__ grid,
+                                                            float4* __restrict__ pbar, DevScene sc, int NB)
 {
     const int fl = fbflag[blockIdx.x];
     if (!cov[blockIdx.x] && !fl) return;
@@ -687,20 +749,17 @@ __global__ void __launch_bounds__(64) k_vbar_reduce(const float4* __restrict__ s
     const int nb3 = NB * NB * NB;
     const int inst = blockIdx.x / nb3, blk = blockIdx.x % nb3;
     const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
-    const int lx = threadIdx.x >> 4, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x & 3;
+    const int node = threadIdx.x & 63;
+    const int lx = node >> 4, ly = (node >> 2) & 3, lz = node & 3;
     const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
     __shared__ int s_nc[3 * 27], s_cs[3 * 27];
     load_neighbour_buckets(s_nc, s_cs, bnch, cstart, inst, bx, by, bz, NB);
     __syncthreads();
-    if (gx >= n || gy >= n || gz >= n) return;
-    const long gid = (long)inst * nn + node_index(n, gx, gy, gz);
-    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (fl) {
-        acc = fallback[gid];
-        fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (threadIdx.x == 0) fbflag[blockIdx.x] = 0;
-    }
-    sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, 3);
+    const bool inb = gx < n && gy < n && gz < n;
+    const long gid = (long)inst * nn + (inb ? node_index(n, gx, gy, gz) : 0);
+    if (fl && threadIdx.x == 0) fbflag[blockIdx.x] = 0;
+    const float4 acc = staged_node_sum<NPART>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 3);
+    if (!inb || threadIdx.x >= 64) return;
     float v[3];
     NodeRec r;
     gridop_node(sc, grid[gid], gx, gy, gz, v, r);

@@ -62,6 +62,9 @@ enum {
                                           or a divisor of it.  Sorting more often keeps drifting particles
                                           inside their chunk's tile; the adjoint replays the forward's sorts.
                                           Reallocates the sort tables: takes effect at the next ds_reset */
+    DS_OPT_REDUCE_PARTS = 11,          /* wavefronts per node block of the tiled grid reductions: 1, 2 or 4
+                                          (default 1); with more, each takes every n-th round of covering
+                                          chunk slots and the partial sums are added in a fixed order */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or

@@ -47,24 +47,36 @@ def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
 
     def timer(engine, cand, substeps, repeats):
         calls.append((dict(cand), substeps))
-        return 1.0 if cand == want else 2.0 + len(calls)
+        kernel_opts = {k: cand[k] for k in want}
+        if kernel_opts != want:
+            return 2.0 + len(calls)
+        # among the later stages' values: sorting once per interval wins, 2 reduction parts win
+        return 1.0 + (0.5 if cand["sort_every"] else 0.0) + (0.0 if cand["reduce_parts"] == 2 else 0.25)
 
     eng = _FakeEngine()
     f = tmp_path / "tune.json"
-    res = A.tune(scenes, ckpt_every=5, engine=eng, timer=timer, cache_file=str(f), warm_steps=0)
-    want2 = dict(want, sort_every=0)   # the second stage keeps sorting once per interval
+    res = A.tune(scenes, ckpt_every=10, engine=eng, timer=timer, cache_file=str(f), warm_steps=0)
+    want2 = dict(want, sort_every=0, reduce_parts=2)
     assert res["options"] == want2
     assert eng.configured[-1] == want2
-    n2 = len(A.sort_candidates(5)) - 1
-    assert len(calls) == len(A.candidates()) + n2 and all(s == 10 for _, s in calls)   # 2 checkpoint segments
-    assert all(c == dict(want, sort_every=R) for (c, _), R in zip(calls[-n2:], A.sort_candidates(5)[1:]))
+    nc = len(A.candidates())
+    n2 = len(A.sort_candidates(10)) - 1
+    assert n2 == 2
+    T = scenes[0].max_steps
+    assert len(calls) == nc + n2 + 2 and all(s == min(T - 1, 20) for _, s in calls)   # 2 checkpoint segments at most
+    # first stage: every kernel candidate with the later options at their defaults
+    assert [c for c, _ in calls[:nc]] == [dict(c, sort_every=0, reduce_parts=1) for c in A.candidates()]
+    # then the sort intervals, then the reduction parts, each with the options chosen so far
+    assert [c for c, _ in calls[nc:nc + n2]] == [dict(want, sort_every=R, reduce_parts=1)
+                                                 for R in A.sort_candidates(10)[1:]]
+    assert [c for c, _ in calls[nc + n2:]] == [dict(want, sort_every=0, reduce_parts=p) for p in (2, 4)]
     # cached in memory and on disk by signature
-    again = A.tune(scenes, ckpt_every=5, engine=eng, timer=lambda *a: pytest.fail("re-timed"))
+    again = A.tune(scenes, ckpt_every=10, engine=eng, timer=lambda *a: pytest.fail("re-timed"))
     assert again["options"] == want2
     disk = json.loads(f.read_text())
     assert disk[res["signature"]]["options"] == want2
     A.clear_cache()
-    from_disk = A.tune(scenes, ckpt_every=5, engine=eng, timer=lambda *a: pytest.fail("re-timed"),
+    from_disk = A.tune(scenes, ckpt_every=10, engine=eng, timer=lambda *a: pytest.fail("re-timed"),
                        cache_file=str(f))
     assert from_disk["options"] == want2
     A.clear_cache()
@@ -76,6 +88,13 @@ def test_sort_candidates_divide_the_interval():
     assert A.sort_candidates(5) == [0]
     for K in (5, 6, 7, 12, 50, 100):
         assert all(K % R == 0 for R in A.sort_candidates(K)[1:])
+
+
+def test_later_stage_defaults_are_the_library_defaults():
+    assert A.later_defaults(50) == {"sort_every": 0, "reduce_parts": 1}
+    assert A.stage_candidates("reduce_parts", 50) == [1, 2, 4]
+    with pytest.raises(KeyError):
+        A.stage_candidates("chunk_grid", 50)
 
 
 def test_signature_separates_workloads():

@@ -245,9 +245,10 @@ def test_autotune_picks_a_candidate_on_the_device():
     res = A.tune([sc], ckpt_every=6, substeps=12, repeats=1)
     opts = dict(res["options"])
     assert opts.pop("sort_every") in A.sort_candidates(6)
+    assert opts.pop("reduce_parts") in (1, 2, 4)
     assert opts in A.candidates()
     assert res["warm_steps"] == min((sc.max_steps - 1) // 2, sc.max_steps - 1 - 12)
-    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1
+    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1 + 2
     assert all(np.isfinite(t["ms"]) and t["ms"] > 0 for t in res["timings_ms"])
     A.clear_cache()
 
@@ -391,3 +392,30 @@ def test_sorting_more_often_gives_the_same_result():
         for k in ("x", "v", "C", "F"):
             np.testing.assert_allclose(sb[k], sa[k], rtol=1e-4, atol=1e-5 * np.abs(sa[k]).max(), err_msg=str(key))
         assert grad_err(db, da) <= 1e-4, key
+
+
+def test_split_grid_reductions_give_the_same_result():
+    """Node-block reductions split over 2 or 4 wavefronts (each sums every n-th round of the
+    covering chunk slots; the partial sums are added in a fixed order) give the one-wavefront
+    forces, gradient and state to FP32 summation-order noise.  Small chunks (4 rows) on a
+    fast-stirred scene give buckets several chunks, so the reductions have several rounds."""
+    sc = S.stirring_scene(32, 150, fill_radius=0.25, fill_height=0.2, stick_radius=0.05, path_rx=0.15, path_rz=0.15,
+                          omega=2 * np.pi * 3, seed=2)
+    res = {}
+    for parts in (1, 2, 4):
+        print(f"[gpu] reduce parts {parts}", flush=True)
+        with Engine([sc], ckpt_every=60, particles_per_thread=1) as e:
+            e.configure(reduce_parts=parts)
+            assert e.options()["reduce_parts"] == parts
+            e.reset()
+            e.forward()
+            F = e.forces()[0]
+            st = e.read_liquid(0)
+            _, g = compute_loss(F, 0.5 * F, 1)
+            res[parts] = (F, st, e.backward(g[None])[0])
+    Fa, sa, da = res[1]
+    for parts, (Fb, sb, db) in res.items():
+        assert force_err(Fb, Fa) <= 1e-5, parts
+        for k in ("x", "v", "C", "F"):
+            np.testing.assert_allclose(sb[k], sa[k], rtol=1e-4, atol=1e-5 * np.abs(sa[k]).max(), err_msg=str(parts))
+        assert grad_err(db, da) <= 1e-4, parts

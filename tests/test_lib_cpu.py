@@ -103,3 +103,22 @@ def test_reference_l1_loss_counts_steps_0_to_T_minus_2():
     L, g = compute_loss(F, np.zeros(T), 0)
     assert L == pytest.approx(sum(range(T - 1)) / (T - 1))
     assert g[T - 1, 0] == 0 and g[1, 0] == pytest.approx(1 / (T - 1))
+
+
+def test_engine_option_numbers_match_the_header():
+    """The Python engine passes the option and value numbers that include/diffstir.h declares."""
+    from diffstir_amd.engine import Engine
+    txt = open(os.path.join(ROOT, "include", "diffstir.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    enum = {k: int(v) for k, v in re.findall(r"\b(DS_[A-Z0-9_]+)\s*=\s*(-?\d+)", txt)}
+    want = {"OPT_KERNELS": "DS_OPT_KERNELS", "OPT_PPT": "DS_OPT_PARTICLES_PER_THREAD",
+            "OPT_TIMING": "DS_OPT_KERNEL_TIMING", "OPT_GATHER_PPT": "DS_OPT_GATHER_PARTICLES_PER_THREAD",
+            "OPT_ADJ_P2G": "DS_OPT_ADJOINT_P2G", "OPT_SCATTER": "DS_OPT_SCATTER_CHUNKS", "OPT_GRAPH": "DS_OPT_GRAPH",
+            "OPT_OVERLAP": "DS_OPT_OVERLAP", "OPT_CHUNK_GRID": "DS_OPT_CHUNK_GRID",
+            "OPT_SORT_EVERY": "DS_OPT_SORT_EVERY", "OPT_REDUCE_PARTS": "DS_OPT_REDUCE_PARTS"}
+    for attr, name in want.items():
+        assert getattr(Engine, attr) == enum[name], name
+    assert len([k for k in enum if k.startswith("DS_OPT_")]) == len(want)
+    assert Engine.KERNELS == {"tiled": enum["DS_KERNELS_TILED"], "per_particle": enum["DS_KERNELS_PER_PARTICLE"]}
+    assert Engine.ADJ_P2G == {"fused": enum["DS_ADJ_P2G_FUSED"], "split": enum["DS_ADJ_P2G_SPLIT"]}
+    assert Engine.SCATTER == {"rows": enum["DS_SCATTER_ROWS"], "packed": enum["DS_SCATTER_PACKED"]}
