@@ -20,7 +20,11 @@ Later stages then refine the chosen set one option at a time:
   n_grid 128 workload) it keeps their stencils inside their chunk's tile;
 * ``reduce_parts`` -- 1, 2 or 4 wavefronts per node block of the grid reductions: where buckets
   hold many chunks (one clumped instance) the rounds of slot loads split over the wavefronts;
-  where they hold one or two (large batches) the extra wavefronts only take up slots.
+  where they hold one or two (large batches) the extra wavefronts only take up slots;
+* ``xgrad_stream`` -- the adjoint P2G's position gradient in its own kernel on a third stream,
+  beside the adjoint G2P and grid reduction of the same step: it shortens the adjoint's kernel
+  chain where the chip has room for a third stream (large batches) and lengthens it where the
+  extra tile load and stencil cost more than the overlap gains.
 
 Which choice is fastest depends on the workload's size and on how the particles cluster, so each
 candidate is timed on the workload itself: a short checkpointed forward + adjoint rollout
@@ -49,7 +53,7 @@ DEFAULT = {"particles_per_thread": 2, "gather_particles_per_thread": 1, "adjoint
            "scatter_chunks": "rows"}
 
 # options refined after the first stage, in order, with their default (first) values
-LATER_STAGES = ("sort_every", "reduce_parts")
+LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream")
 
 _CACHE: Dict[str, dict] = {}
 
@@ -84,6 +88,8 @@ def stage_candidates(key: str, ckpt_every: int) -> List:
         return sort_candidates(ckpt_every)
     if key == "reduce_parts":
         return [1, 2, 4]
+    if key == "xgrad_stream":
+        return [False, True]
     raise KeyError(key)
 
 
@@ -129,7 +135,7 @@ def warmed_scenes(scenes, engine, warm_steps: int):
     options) reaches after `warm_steps` substeps: liquid and stick positions of that step, the
     stick trajectory from that frame on."""
     import dataclasses
-    engine.configure(**DEFAULT, sort_every=0, reduce_parts=1)
+    engine.configure(**DEFAULT, sort_every=0, reduce_parts=1, xgrad_stream=False)
     engine.reset()
     engine.forward(warm_steps)
     out = []

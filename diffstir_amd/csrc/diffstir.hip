@@ -131,6 +131,11 @@ struct ds_ctx {
     bool overlap_alloc = false;
     hipStream_t stream2 = nullptr;     // the recompute stream (default priority, like the main stream)
     hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr};
+    bool xgrad_stream = false;         // adjoint P2G's position gradient on its own stream (DS_OPT_XGRAD_STREAM)
+    bool xgrad_alloc = false;
+    hipStream_t stream3 = nullptr;     // the position-gradient stream (default priority)
+    hipEvent_t ev_step = nullptr, ev_xg = nullptr;
+    float* d_xp = nullptr;             // [3][P] position gradient of the current adjoint step
     int* d_brows = nullptr;       // rows per bucket
     int* d_rstart = nullptr;      // first row of each bucket
     int* d_src = nullptr;         // ncp x P
@@ -354,6 +359,8 @@ void free_overlap(ds_ctx* c)
     dfree(c->d_fbflag_r);
     dfree(c->d_seg_tmp_b);
     c->overlap_alloc = false;
+    dfree(c->d_xp);
+    c->xgrad_alloc = false;
 }
 
 void free_mode(ds_ctx* c)
@@ -433,14 +440,16 @@ void free_all(ds_ctx* c)
     dfree(c->d_flag);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    for (hipEvent_t* e : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_adj[0], &c->ev_adj[1]}) {
+    for (hipEvent_t* e : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_adj[0], &c->ev_adj[1], &c->ev_step,
+                          &c->ev_xg}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
+    if (c->stream3) (void)hipStreamDestroy(c->stream3);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->ev0 = c->ev1 = nullptr;
-    c->stream = c->stream2 = nullptr;
+    c->stream = c->stream2 = c->stream3 = nullptr;
 }
 
 // Buffers that depend on the kernel family, allocated at the first ds_reset after a change.
@@ -522,6 +531,19 @@ int alloc_mode(ds_ctx* c)
     }
 #undef MALLOC
     c->alloc_mode = c->mode;
+    return DS_OK;
+}
+
+// The position-gradient buffer, stream and events of the adjoint (tiled family), allocated at
+// the first ds_backward that uses them.
+int alloc_xgrad(ds_ctx* c)
+{
+    if (c->xgrad_alloc || c->mode != DS_KERNELS_TILED) return DS_OK;
+    RCHK(dalloc(c, &c->d_xp, (size_t)3 * c->P));
+    if (!c->stream3) HIPCHK(c, hipStreamCreate(&c->stream3));
+    for (hipEvent_t* e : {&c->ev_step, &c->ev_xg})
+        if (!*e) HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    c->xgrad_alloc = true;
     return DS_OK;
 }
 
@@ -706,6 +728,12 @@ inline dim3 chunk_grid(const ds_ctx* c, long maxchunks)
         else if ((ppt) == 2) DS_LAUNCH_T(kt, kern<2>, __VA_ARGS__);                              \
         else DS_LAUNCH_T(kt, kern<4>, __VA_ARGS__);                                              \
     } while (0)
+#define DS_PPT_DISPATCH3_T(kt, ppt, kern, b, b2, ...)                                            \
+    do {                                                                                         \
+        if ((ppt) == 1) DS_LAUNCH_T(kt, (kern<1, b, b2>), __VA_ARGS__);                          \
+        else if ((ppt) == 2) DS_LAUNCH_T(kt, (kern<2, b, b2>), __VA_ARGS__);                     \
+        else DS_LAUNCH_T(kt, (kern<4, b, b2>), __VA_ARGS__);                                     \
+    } while (0)
 #define DS_PPT_DISPATCH2_T(kt, ppt, kern, b, ...)                                                \
     do {                                                                                         \
         if ((ppt) == 1) DS_LAUNCH_T(kt, (kern<1, b>), __VA_ARGS__);                              \
@@ -882,7 +910,7 @@ void recompute_segment(ds_ctx* c, const Exec& x, int cp, int S, const SegSet& z)
 // Adjoint of checkpoint segment cp from its recomputed buffer set z, last step first, on the
 // main stream; then the segment's parameter-gradient partials into d_thbar and the particle
 // adjoint back into the previous checkpoint's order.
-void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
+int adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
 {
     const long SF = c->stateFloats();
     const long G = c->gridNodes() * c->B;
@@ -898,6 +926,12 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
     auto aff_of = [&](int k) -> float* { return z.aff + (size_t)k * 9 * c->P; };
     auto mj_of = [&](int k) -> double* { return z.mj + (size_t)k * 10 * c->P; };
     fill_zero_f64(c, c->d_thpart, 4L * c->maxgch);
+    // the position gradient of step k runs on stream3 beside the adjoint G2P and grid reduction
+    // of step k: it waits for everything the main stream did before (ev_step: the previous step's
+    // adjoint P2G wrote the particle adjoint it reads and read d_xp), and the adjoint P2G of step
+    // k waits for it (ev_xg)
+    const bool xs = c->xgrad_stream && c->xgrad_alloc;
+    if (xs) HIPCHK(c, hipEventRecord(c->ev_step, c->stream));
     for (int k = L - 1; k >= 0; k--) {
         const int s = s0 + k;
         // the tables of the sort this step ran under
@@ -909,6 +943,13 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
         const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)slot * c->NBK;
         const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)slot * c->NBK;
         c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
+        if (xs) {
+            HIPCHK(c, hipStreamWaitEvent(c->stream3, c->ev_step, 0));
+            KernelTimer kt(c, DS_K_P2G_ADJ_XGRAD, true);
+            DS_PPT_DISPATCH_T(kt, c->gppt_sorted, k_p2g_adj_xgrad, ggrid, dim3(WG), 0, c->stream3, st_of(k), c->d_adj,
+                              gchunks, counts + 3, maxgch, grid_of(k), c->d_xp, c->dsc, c->P);
+            HIPCHK(c, hipEventRecord(c->ev_xg, c->stream3));
+        }
         {
             KernelTimer kt(c, DS_K_G2P_ADJ, true);
             if (c->packed)
@@ -926,12 +967,20 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
                                 cstart, c->d_cov + (size_t)slot * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k),
                                 c->d_pbar2, c->dsc, c->NB);
         }
+        if (xs) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_xg, 0));
         if (c->adj_split) {
             {
                 KernelTimer kt(c, DS_K_P2G_ADJ, true);
-                DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_p2g_adj_tiled, false, ggrid, dim3(WG), 0, c->stream,
-                                   st_of(k), aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k),
-                                   c->d_pbar2, c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
+                if (xs)
+                    DS_PPT_DISPATCH3_T(kt, c->gppt_sorted, k_p2g_adj_tiled, false, true, ggrid, dim3(WG), 0, c->stream,
+                                       st_of(k), aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k),
+                                       c->d_pbar2, c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->d_xp, c->dsc,
+                                       c->P);
+                else
+                    DS_PPT_DISPATCH3_T(kt, c->gppt_sorted, k_p2g_adj_tiled, false, false, ggrid, dim3(WG), 0, c->stream,
+                                       st_of(k), aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k),
+                                       c->d_pbar2, c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->d_xp, c->dsc,
+                                       c->P);
             }
             KernelTimer kt(c, DS_K_P2G_ADJ_STRESS, true);
             DS_PPT_DISPATCH_T(kt, c->gppt_sorted, k_p2g_adj_stress, ggrid, dim3(WG), 0, c->stream, st_of(k), c->d_adj,
@@ -939,11 +988,19 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
                               c->P);
         } else {
             KernelTimer kt(c, DS_K_P2G_ADJ, true);
-            DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_p2g_adj_tiled, true, ggrid, dim3(WG), 0, c->stream, st_of(k),
-                               aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2, c->d_gadj,
-                               c->d_params, mj_of(k), c->d_thpart, c->dsc, c->P);
+            if (xs)
+                DS_PPT_DISPATCH3_T(kt, c->gppt_sorted, k_p2g_adj_tiled, true, true, ggrid, dim3(WG), 0, c->stream,
+                                   st_of(k), aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2,
+                                   c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->d_xp, c->dsc, c->P);
+            else
+                DS_PPT_DISPATCH3_T(kt, c->gppt_sorted, k_p2g_adj_tiled, true, false, ggrid, dim3(WG), 0, c->stream,
+                                   st_of(k), aff_of(k), c->d_adj, gchunks, counts + 3, maxgch, grid_of(k), c->d_pbar2,
+                                   c->d_gadj, c->d_params, mj_of(k), c->d_thpart, c->d_xp, c->dsc, c->P);
         }
-        if (k % R != 0) continue;
+        if (k % R != 0) {
+            if (xs) HIPCHK(c, hipEventRecord(c->ev_step, c->stream));
+            continue;
+        }
         // first step after a sort: the sort's parameter-gradient partials, and the adjoint back
         // into the particle order before that sort (none before the very first one)
         hipLaunchKernelGGL(k_thpart_reduce, dim3((unsigned)std::min<long>((c->maxgch + 255) / 256, 1024)), dim3(256),
@@ -955,7 +1012,9 @@ void adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
             hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)std::min<long>(nblocks(SF), 8192)), dim3(256), 0,
                                c->stream, c->d_adj, c->d_adj2, SF);
         }
+        if (xs) HIPCHK(c, hipEventRecord(c->ev_step, c->stream));
     }
+    return DS_OK;
 }
 
 // The checkpointed adjoint of the recorded rollout.  Sequential: recompute segment k, then its
@@ -971,7 +1030,7 @@ int backward_tiled(ds_ctx* c)
     if (!c->overlap || !c->overlap_alloc) {
         for (int cp = last; cp >= 0; cp--) {
             recompute_segment(c, main_exec(c), cp, S, seg_set(c, 0));
-            adjoint_segment(c, cp, S, seg_set(c, 0));
+            RCHK(adjoint_segment(c, cp, S, seg_set(c, 0)));
         }
         return DS_OK;
     }
@@ -988,7 +1047,7 @@ int backward_tiled(ds_ctx* c)
             HIPCHK(c, hipEventRecord(c->ev_rec[(cp - 1) & 1], c->stream2));
         }
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_rec[cp & 1], 0));
-        adjoint_segment(c, cp, S, seg_set(c, cp & 1));
+        RCHK(adjoint_segment(c, cp, S, seg_set(c, cp & 1)));
         HIPCHK(c, hipEventRecord(c->ev_adj[cp & 1], c->stream));
     }
     return DS_OK;
@@ -1195,6 +1254,12 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         c->sort_every = v;
         return DS_OK;
     }
+    if (option == DS_OPT_XGRAD_STREAM) {
+        if (value != 0 && value != 1) return fail(c, DS_EINVAL, "DS_OPT_XGRAD_STREAM takes 0 or 1");
+        if (c->xgrad_stream != (value == 1)) drop_graphs(c);
+        c->xgrad_stream = value == 1;
+        return DS_OK;
+    }
     if (option == DS_OPT_REDUCE_PARTS) {
         if (value != 1 && value != 2 && value != 4) return fail(c, DS_EINVAL, "DS_OPT_REDUCE_PARTS takes 1, 2 or 4");
         if (c->rparts != value) drop_graphs(c);
@@ -1365,6 +1430,7 @@ int ds_backward(ds_ctx* c, const double* dl_dforces, double* dl_dparams)
     for (size_t i = 0; i < dvb.size(); i++) dvb[i] = scale * dl_dforces[i];
     HIPCHK(c, hipMemcpyAsync(c->d_dvbar, dvb.data(), sizeof(double) * dvb.size(), hipMemcpyHostToDevice, c->stream));
     if (c->mode == DS_KERNELS_TILED && c->overlap) RCHK(alloc_overlap(c));
+    if (c->mode == DS_KERNELS_TILED && c->xgrad_stream) RCHK(alloc_xgrad(c));
     const bool graph = c->use_graph && c->mode == DS_KERNELS_TILED && c->step == c->T - 1;
     if (graph && !c->bwd_exec) RCHK(capture_graph(c, backward_tiled_n, 0, &c->bwd_exec));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -824,14 +824,17 @@ __device__ __forceinline__ void add_chunk_partials(double (*red)[4], const doubl
 // of this step's velocity) and the gather of dL/d(momentum, mass) with the stored affine (ub, Ab,
 // fxb); for stick and container particles their whole adjoint (dL/dC).  kFused: the stress half
 // follows in the same thread (p2g_adj_stress_at, partials into thpart); otherwise the gathered
-// quantities go to gout ([18][P]: xp, ub, Ab, fxb) for k_p2g_adj_stress.
-template <int PPT, bool kFused>
+// quantities go to gout ([18][P]: xp, ub, Ab, fxb) for k_p2g_adj_stress.  kXp: the position
+// gradient through G2P's gather was computed by k_p2g_adj_xgrad into xpin ([3][P]), so neither it
+// nor the velocity tile is computed here.
+template <int PPT, bool kFused, bool kXp>
 __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ st, const float* __restrict__ ain,
                                                       float* __restrict__ adj, const GChunk* __restrict__ chunks,
                                                       const int* __restrict__ nchunks_p, int maxchunks, const float4* __restrict__ grid,
                                                       const float4* __restrict__ pbar, float* __restrict__ gout,
                                                       const double* __restrict__ params, const double* __restrict__ mj,
-                                                      double* __restrict__ thpart, DevScene sc, int P)
+                                                      double* __restrict__ thpart, const float* __restrict__ xpin,
+                                                      DevScene sc, int P)
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
     const int nchunks = min(*nchunks_p, maxchunks);
@@ -857,7 +860,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             tq[3 * TILE + id] = v.w;
         }
         const float4* gi = grid + (long)ch.inst * nn;
-        if (ch.species == LIQUID) load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        if (!kXp && ch.species == LIQUID) load_velocity_tile(tv, gi, sc, x0, y0, z0);
         __syncthreads();
         double tb[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
@@ -873,7 +876,14 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             const bool it = stencil_in_tile(s, x0, y0, z0);
             if (ch.species == LIQUID) {
                 float xp[3];
-                {
+                if (kXp) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++) xp[a] = xpin[a * LP + p];
+                    if (!kFused) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) gout[a * LP + p] = xp[a];
+                    }
+                } else {
                     // the G2P part of dL/dx (its weights and offsets depend on x): dL/dx of the
                     // next step plus the gradient through the gather of this step's velocity
                     float xa[3], vt[3], Cb[9];
@@ -937,6 +947,59 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             }
         }
         if (kFused && ch.species == LIQUID) add_chunk_partials(red, tb, thpart, c);
+        __syncthreads();
+    }
+}
+
+// The position gradient of the adjoint P2G on its own, for liquid particles: dL/dx of the next
+// step plus the gradient through G2P's gather of this step's velocity (weights and offsets
+// depend on x), into xp ([3][P]).  It needs neither dL/d(momentum, mass) nor the stress, so it
+// runs beside the adjoint G2P and grid reduction of the same step, on another stream.
+template <int PPT>
+__global__ void __launch_bounds__(WG) k_p2g_adj_xgrad(const float* __restrict__ st, const float* __restrict__ adj,
+                                                      const GChunk* __restrict__ chunks, const int* __restrict__ nchunks_p,
+                                                      int maxchunks, const float4* __restrict__ grid,
+                                                      float* __restrict__ xp, DevScene sc, int P)
+{
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
+    __shared__ float tv[3 * TILE];
+    const long nn = (long)sc.n_grid * sc.n_grid * sc.n_grid;
+    const long LP = P;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const GChunk ch = chunks[c];
+        if (ch.species != LIQUID) continue;   // uniform across the workgroup
+        const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
+        const float4* gi = grid + (long)ch.inst * nn;
+        load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        __syncthreads();
+#pragma unroll 1
+        for (int r = 0; r < PPT; r++) {
+            const int idx = r * WG + threadIdx.x;
+            if (idx >= ch.count) break;
+            const int p = ch.start + idx;
+            float x[3], xa[3], vt[3], Cb[9], g[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
+            Stencil s;
+            stencil(sc, x[0], x[1], x[2], s);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                xa[a] = adj[(SX + a) * LP + p];
+                vt[a] = adj[(SV + a) * LP + p] + sc.dt * xa[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
+            if (stencil_in_tile(s, x0, y0, z0)) {
+                TileV gv{tv, x0, y0, z0};
+                g2p_adj_xgrad(sc, s, vt, Cb, gv, g);
+            } else {
+                GatherVfromP gv{gi, sc};
+                g2p_adj_xgrad<true>(sc, s, vt, Cb, gv, g);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) xp[a * LP + p] = g[a] + xa[a];
+        }
         __syncthreads();
     }
 }

@@ -46,7 +46,7 @@ class Engine:
 
     # execution options of the tiled kernels (ds_set_option ids)
     OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
-    OPT_OVERLAP, OPT_CHUNK_GRID, OPT_SORT_EVERY, OPT_REDUCE_PARTS = 8, 9, 10, 11
+    OPT_OVERLAP, OPT_CHUNK_GRID, OPT_SORT_EVERY, OPT_REDUCE_PARTS, OPT_XGRAD_STREAM = 8, 9, 10, 11, 12
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
@@ -85,6 +85,7 @@ class Engine:
                        scatter_chunks=scatter_chunks, overlap=overlap)
         self.sort_every = 0
         self.reduce_parts = 1
+        self.xgrad_stream = False
         self.set_graph(graph)
         for b, s in enumerate(scenes):
             self.upload_scene(b, s)
@@ -107,14 +108,17 @@ class Engine:
     def configure(self, particles_per_thread: Optional[int] = None, gather_particles_per_thread: Optional[int] = None,
                   adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None,
                   overlap: Optional[bool] = None, chunk_grid: Optional[int] = None,
-                  sort_every: Optional[int] = None, reduce_parts: Optional[int] = None):
+                  sort_every: Optional[int] = None, reduce_parts: Optional[int] = None,
+                  xgrad_stream: Optional[bool] = None):
         """Execution options of the tiled kernels (they change how the substep runs, never what it
         computes); particle-per-thread changes take effect at the next reset.  `overlap`: the
         adjoint recomputes the next checkpoint segment on a second stream while the current one's
         adjoint runs (a second set of segment buffers).  `chunk_grid`: workgroups per chunk-kernel
         launch (0: the library default).  `sort_every`: substeps between bucket sorts (0: the
         checkpoint interval; else a divisor of it), from the next reset.  `reduce_parts`:
-        wavefronts per node block of the grid reductions (1, 2 or 4)."""
+        wavefronts per node block of the grid reductions (1, 2 or 4).  `xgrad_stream`: the adjoint
+        P2G's position gradient in its own kernel on a third stream, beside the adjoint G2P and
+        grid reduction of the same step."""
         if particles_per_thread is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_PPT, int(particles_per_thread)))
             self.particles_per_thread = int(particles_per_thread)
@@ -142,6 +146,9 @@ class Engine:
         if reduce_parts is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_REDUCE_PARTS, int(reduce_parts)))
             self.reduce_parts = int(reduce_parts)
+        if xgrad_stream is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_XGRAD_STREAM, 1 if xgrad_stream else 0))
+            self.xgrad_stream = bool(xgrad_stream)
 
     def set_kernel_timing(self, every: int):
         """Sample every N-th substep's kernels with HIP events (0: off)."""
@@ -157,7 +164,7 @@ class Engine:
         return {"particles_per_thread": self.particles_per_thread,
                 "gather_particles_per_thread": self.gather_particles_per_thread, "adjoint_p2g": self.adjoint_p2g,
                 "scatter_chunks": self.scatter_chunks, "overlap": self.overlap, "sort_every": self.sort_every,
-                "reduce_parts": self.reduce_parts}
+                "reduce_parts": self.reduce_parts, "xgrad_stream": self.xgrad_stream}
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):

@@ -65,6 +65,10 @@ enum {
     DS_OPT_REDUCE_PARTS = 11,          /* wavefronts per node block of the tiled grid reductions: 1, 2 or 4
                                           (default 1); with more, each takes every n-th round of covering
                                           chunk slots and the partial sums are added in a fixed order */
+    DS_OPT_XGRAD_STREAM = 12,          /* tiled adjoint: 1 computes the adjoint P2G's position gradient
+                                          (through G2P's gather) in its own kernel on a third stream,
+                                          beside the adjoint G2P and grid reduction of the same step;
+                                          0 (default): inside the adjoint P2G */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or
@@ -82,7 +86,8 @@ enum {
     DS_K_SORT = 6,         /* the whole bucket sort of one checkpoint */
     DS_K_P2G_ADJ_STRESS = 7, /* k_p2g_adj_stress: adjoint of the stress and of the FP64 SVD */
     DS_K_G2P_FACTOR = 8,   /* k_g2p_tiled of the recompute, with the full FP64 SVD */
-    DS_NKERNELS = 9
+    DS_K_P2G_ADJ_XGRAD = 9, /* k_p2g_adj_xgrad: the adjoint P2G's position gradient */
+    DS_NKERNELS = 10
 };
 enum { DS_KERNELS_TILED = 0, DS_KERNELS_PER_PARTICLE = 1 };
 

@@ -246,9 +246,10 @@ def test_autotune_picks_a_candidate_on_the_device():
     opts = dict(res["options"])
     assert opts.pop("sort_every") in A.sort_candidates(6)
     assert opts.pop("reduce_parts") in (1, 2, 4)
+    assert opts.pop("xgrad_stream") in (False, True)
     assert opts in A.candidates()
     assert res["warm_steps"] == min((sc.max_steps - 1) // 2, sc.max_steps - 1 - 12)
-    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1 + 2
+    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1 + 2 + 1
     assert all(np.isfinite(t["ms"]) and t["ms"] > 0 for t in res["timings_ms"])
     A.clear_cache()
 
@@ -419,3 +420,34 @@ def test_split_grid_reductions_give_the_same_result():
         for k in ("x", "v", "C", "F"):
             np.testing.assert_allclose(sb[k], sa[k], rtol=1e-4, atol=1e-5 * np.abs(sa[k]).max(), err_msg=str(parts))
         assert grad_err(db, da) <= 1e-4, parts
+
+
+def test_position_gradient_stream_gives_the_same_gradient():
+    """The adjoint P2G's position gradient computed by its own kernel on a third stream, beside
+    the adjoint G2P and grid reduction of the same step, gives the gradient computed inside the
+    adjoint P2G: launched and replayed as a hipGraph, with and without the overlapped recompute,
+    with one sort per checkpoint interval and with sorts inside it."""
+    sc = S.stirring_scene(32, 150, fill_radius=0.25, fill_height=0.2, stick_radius=0.05, path_rx=0.15, path_rz=0.15,
+                          omega=2 * np.pi * 3, seed=2)
+    res = {}
+    for xs, graph, overlap, R in ((False, False, True, 0), (True, False, True, 0), (True, True, True, 0),
+                                  (False, False, False, 30), (True, False, False, 30), (True, True, True, 30)):
+        print(f"[gpu] position-gradient stream {xs}, graph {graph}, overlap {overlap}, sort every {R or 60}",
+              flush=True)
+        with Engine([sc], ckpt_every=60, graph=graph, overlap=overlap) as e:
+            e.configure(sort_every=R, xgrad_stream=xs)
+            assert e.options()["xgrad_stream"] == xs
+            for rep in range(2):
+                e.reset()
+                e.forward()
+                F = e.forces()[0]
+                _, g = compute_loss(F, 0.5 * F, 1)
+                d = e.backward(g[None])[0]
+            res[(xs, graph, overlap, R)] = (F, d)
+    for R in (0, 30):
+        Fa, da = res[(False, False, R == 0, R)]
+        for key, (Fb, db) in res.items():
+            if key[3] != R:
+                continue
+            assert force_err(Fb, Fa) <= 1e-6, key
+            assert grad_err(db, da) <= 1e-6, key

@@ -115,10 +115,12 @@ def test_engine_option_numbers_match_the_header():
             "OPT_TIMING": "DS_OPT_KERNEL_TIMING", "OPT_GATHER_PPT": "DS_OPT_GATHER_PARTICLES_PER_THREAD",
             "OPT_ADJ_P2G": "DS_OPT_ADJOINT_P2G", "OPT_SCATTER": "DS_OPT_SCATTER_CHUNKS", "OPT_GRAPH": "DS_OPT_GRAPH",
             "OPT_OVERLAP": "DS_OPT_OVERLAP", "OPT_CHUNK_GRID": "DS_OPT_CHUNK_GRID",
-            "OPT_SORT_EVERY": "DS_OPT_SORT_EVERY", "OPT_REDUCE_PARTS": "DS_OPT_REDUCE_PARTS"}
+            "OPT_SORT_EVERY": "DS_OPT_SORT_EVERY", "OPT_REDUCE_PARTS": "DS_OPT_REDUCE_PARTS",
+            "OPT_XGRAD_STREAM": "DS_OPT_XGRAD_STREAM"}
     for attr, name in want.items():
         assert getattr(Engine, attr) == enum[name], name
     assert len([k for k in enum if k.startswith("DS_OPT_")]) == len(want)
     assert Engine.KERNELS == {"tiled": enum["DS_KERNELS_TILED"], "per_particle": enum["DS_KERNELS_PER_PARTICLE"]}
     assert Engine.ADJ_P2G == {"fused": enum["DS_ADJ_P2G_FUSED"], "split": enum["DS_ADJ_P2G_SPLIT"]}
     assert Engine.SCATTER == {"rows": enum["DS_SCATTER_ROWS"], "packed": enum["DS_SCATTER_PACKED"]}
+    assert len(_lib.KERNEL_NAMES) == enum["DS_NKERNELS"]
