@@ -24,7 +24,10 @@ Later stages then refine the chosen set one option at a time:
 * ``xgrad_stream`` -- the adjoint P2G's position gradient in its own kernel on a third stream,
   beside the adjoint G2P and grid reduction of the same step: it shortens the adjoint's kernel
   chain where the chip has room for a third stream (large batches) and lengthens it where the
-  extra tile load and stencil cost more than the overlap gains.
+  extra tile load and stencil cost more than the overlap gains;
+* ``g2p_split`` -- the forward's G2P running each gather chunk as 2 or 4 workgroups: with 2 gather
+  particles per thread chosen for the adjoint P2G, the forward's G2P can still run 256-particle
+  workgroups (only values up to the chosen particles per thread are tried).
 
 Which choice is fastest depends on the workload's size and on how the particles cluster, so each
 candidate is timed on the workload itself: a short checkpointed forward + adjoint rollout
@@ -53,7 +56,7 @@ DEFAULT = {"particles_per_thread": 2, "gather_particles_per_thread": 1, "adjoint
            "scatter_chunks": "rows"}
 
 # options refined after the first stage, in order, with their default (first) values
-LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream")
+LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream", "g2p_split")
 
 _CACHE: Dict[str, dict] = {}
 
@@ -82,14 +85,18 @@ def sort_candidates(ckpt_every: int) -> List[int]:
     return [0] + [K // d for d in (2, 4, 5) if K % d == 0 and K // d >= 2][:2]
 
 
-def stage_candidates(key: str, ckpt_every: int) -> List:
-    """Values of a later-stage option, its default first."""
+def stage_candidates(key: str, ckpt_every: int, chosen: Optional[dict] = None) -> List:
+    """Values of a later-stage option, its default first (`chosen`: the options chosen so far,
+    which rule out values that would change nothing)."""
     if key == "sort_every":
         return sort_candidates(ckpt_every)
     if key == "reduce_parts":
         return [1, 2, 4]
     if key == "xgrad_stream":
         return [False, True]
+    if key == "g2p_split":
+        g = (chosen or {}).get("gather_particles_per_thread", 4)
+        return [1] + [v for v in (2, 4) if v <= g]
     raise KeyError(key)
 
 
@@ -135,7 +142,7 @@ def warmed_scenes(scenes, engine, warm_steps: int):
     options) reaches after `warm_steps` substeps: liquid and stick positions of that step, the
     stick trajectory from that frame on."""
     import dataclasses
-    engine.configure(**DEFAULT, sort_every=0, reduce_parts=1, xgrad_stream=False)
+    engine.configure(**DEFAULT, sort_every=0, reduce_parts=1, xgrad_stream=False, g2p_split=1)
     engine.reset()
     engine.forward(warm_steps)
     out = []
@@ -194,7 +201,7 @@ def tune(scenes, ckpt_every: int = 50, substeps: Optional[int] = None, space: Op
         best_t = min(t for c, t in timings if c == best)
         # later stages: one option at a time, with the options chosen so far
         for key in LATER_STAGES:
-            vals = stage_candidates(key, ckpt_every)
+            vals = stage_candidates(key, ckpt_every, best)
             stage = [(best, best_t)] + [(dict(best, **{key: v}), None) for v in vals if v != best[key]]
             stage = [(c, t if t is not None else run(c)) for c, t in stage]
             best = pick(stage)

@@ -47,6 +47,7 @@ struct ds_ctx {
     int gppt_sorted = 1;
     bool adj_split = false;            // adjoint P2G as two kernels (gather, stress) instead of one
     int rparts = 1;                    // wavefronts per node block of the grid reductions (1, 2, 4)
+    int g2p_split = 1;                 // workgroups per gather chunk in G2P (DS_OPT_G2P_SPLIT)
     bool packed = false;               // scatter kernels on the packed (gather) chunks with claim passes
     bool use_graph = false;            // replay whole-rollout forward / adjoint launch sequences as hipGraphs
     bool overlap = true;               // adjoint: recompute segment k-1 on a second stream during segment k's adjoint
@@ -793,14 +794,18 @@ void g2p_tiled(ds_ctx* c, const Exec& x, const float* in, float* out, int cp, in
     const GChunk* gch = c->d_gchunks + (size_t)cp * c->maxgch;
     const int* ngc = c->d_counts + 4 * cp + 3;
     double* dv = record ? c->d_dvsum : nullptr;
+    // the forward's chunks of gppt particles per thread as `split` workgroups of gppt / split
+    // each (the recompute's G2P keeps whole chunks: there a faster G2P only competes harder with
+    // the adjoint on the other stream)
+    const int split = mj ? 1 : std::max(1, std::min(c->g2p_split, c->gppt_sorted));
+    const int ppt = c->gppt_sorted / split;
+    const dim3 grid = chunk_grid(c, c->maxgch * split);
     if (mj)
-        DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_g2p_tiled, true, chunk_grid(c, c->maxgch), dim3(WG), 0, x.stream, in,
-                           out, gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc,
-                           c->P, c->d_flag);
+        DS_PPT_DISPATCH2_T(kt, ppt, k_g2p_tiled, true, grid, dim3(WG), 0, x.stream, in, out, gch, ngc, (int)c->maxgch,
+                           c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P, c->d_flag, split);
     else
-        DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_g2p_tiled, false, chunk_grid(c, c->maxgch), dim3(WG), 0, x.stream, in,
-                           out, gch, ngc, (int)c->maxgch, c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc,
-                           c->P, c->d_flag);
+        DS_PPT_DISPATCH2_T(kt, ppt, k_g2p_tiled, false, grid, dim3(WG), 0, x.stream, in, out, gch, ngc, (int)c->maxgch,
+                           c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P, c->d_flag, split);
 }
 
 // J of every liquid particle of the checkpoint state `st` into d_j, for the segment's first P2G
@@ -1252,6 +1257,12 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
             c->reset_done = false;
         }
         c->sort_every = v;
+        return DS_OK;
+    }
+    if (option == DS_OPT_G2P_SPLIT) {
+        if (value != 1 && value != 2 && value != 4) return fail(c, DS_EINVAL, "DS_OPT_G2P_SPLIT takes 1, 2 or 4");
+        if (c->g2p_split != value) drop_graphs(c);
+        c->g2p_split = (int)value;
         return DS_OK;
     }
     if (option == DS_OPT_XGRAD_STREAM) {

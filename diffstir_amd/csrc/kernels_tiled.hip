@@ -473,13 +473,15 @@ __global__ void __launch_bounds__(64 * NPART) k_grid_reduce(const float4* __rest
 
 // kFactor (the recompute of the adjoint): the full FP64 SVD, storing J and its SVD backward
 // factor M for the adjoint P2G of the next step in mj ([10][P]: M row-major, then J).
+// split: workgroups per chunk; part h of a chunk takes its particles [h, h + 1) * WG * PPT, so a
+// table of 512-particle chunks runs as 256-particle workgroups (each loads the chunk's tile).
 template <int PPT, bool kFactor>
 __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, float* __restrict__ out,
                                                   const GChunk* __restrict__ chunks, const int* __restrict__ nchunks_p, int maxchunks,
                                                   const float4* __restrict__ vgrid, const float* __restrict__ traj,
                                                   int T, int f, double* __restrict__ dvsum, double* __restrict__ jout,
                                                   double* __restrict__ mj, DevScene sc, int P,
-                                                  unsigned* __restrict__ flag)
+                                                  unsigned* __restrict__ flag, int split)
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
     const int nchunks = min(*nchunks_p, maxchunks);
@@ -488,8 +490,10 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
     const long LP = P;
-    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    for (int w = blockIdx.x; w < nchunks * split; w += gridDim.x) {
+        const int c = w / split, base = (w % split) * (WG * PPT);
         const GChunk ch = chunks[c];
+        if (base >= ch.count) continue;   // uniform across the workgroup
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const float4* vi = vgrid + (long)ch.inst * nn;
         for (int e = threadIdx.x; e < TNODES; e += WG) {
@@ -507,7 +511,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
         const float* tr = traj + ((long)ch.inst * T + f) * 3;
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
-            const int idx = r * WG + threadIdx.x;
+            const int idx = base + r * WG + threadIdx.x;
             if (idx >= ch.count) break;
             const int p = ch.start + idx;
             float x[3], Fo[9];

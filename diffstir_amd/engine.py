@@ -47,6 +47,7 @@ class Engine:
     # execution options of the tiled kernels (ds_set_option ids)
     OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
     OPT_OVERLAP, OPT_CHUNK_GRID, OPT_SORT_EVERY, OPT_REDUCE_PARTS, OPT_XGRAD_STREAM = 8, 9, 10, 11, 12
+    OPT_G2P_SPLIT = 13
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
@@ -86,6 +87,7 @@ class Engine:
         self.sort_every = 0
         self.reduce_parts = 1
         self.xgrad_stream = False
+        self.g2p_split = 1
         self.set_graph(graph)
         for b, s in enumerate(scenes):
             self.upload_scene(b, s)
@@ -109,7 +111,7 @@ class Engine:
                   adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None,
                   overlap: Optional[bool] = None, chunk_grid: Optional[int] = None,
                   sort_every: Optional[int] = None, reduce_parts: Optional[int] = None,
-                  xgrad_stream: Optional[bool] = None):
+                  xgrad_stream: Optional[bool] = None, g2p_split: Optional[int] = None):
         """Execution options of the tiled kernels (they change how the substep runs, never what it
         computes); particle-per-thread changes take effect at the next reset.  `overlap`: the
         adjoint recomputes the next checkpoint segment on a second stream while the current one's
@@ -118,7 +120,8 @@ class Engine:
         checkpoint interval; else a divisor of it), from the next reset.  `reduce_parts`:
         wavefronts per node block of the grid reductions (1, 2 or 4).  `xgrad_stream`: the adjoint
         P2G's position gradient in its own kernel on a third stream, beside the adjoint G2P and
-        grid reduction of the same step."""
+        grid reduction of the same step.  `g2p_split`: workgroups per gather chunk in the forward's G2P
+        (1, 2 or 4, at most the gather particles per thread)."""
         if particles_per_thread is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_PPT, int(particles_per_thread)))
             self.particles_per_thread = int(particles_per_thread)
@@ -149,6 +152,9 @@ class Engine:
         if xgrad_stream is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_XGRAD_STREAM, 1 if xgrad_stream else 0))
             self.xgrad_stream = bool(xgrad_stream)
+        if g2p_split is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_G2P_SPLIT, int(g2p_split)))
+            self.g2p_split = int(g2p_split)
 
     def set_kernel_timing(self, every: int):
         """Sample every N-th substep's kernels with HIP events (0: off)."""
@@ -164,7 +170,7 @@ class Engine:
         return {"particles_per_thread": self.particles_per_thread,
                 "gather_particles_per_thread": self.gather_particles_per_thread, "adjoint_p2g": self.adjoint_p2g,
                 "scatter_chunks": self.scatter_chunks, "overlap": self.overlap, "sort_every": self.sort_every,
-                "reduce_parts": self.reduce_parts, "xgrad_stream": self.xgrad_stream}
+                "reduce_parts": self.reduce_parts, "xgrad_stream": self.xgrad_stream, "g2p_split": self.g2p_split}
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):

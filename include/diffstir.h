@@ -69,6 +69,10 @@ enum {
                                           (through G2P's gather) in its own kernel on a third stream,
                                           beside the adjoint G2P and grid reduction of the same step;
                                           0 (default): inside the adjoint P2G */
+    DS_OPT_G2P_SPLIT = 13,             /* workgroups per gather chunk in the forward's G2P: 1 (default), 2
+                                          or 4 (at most the gather particles per thread): with 2 particles
+                                          per thread the adjoint P2G runs 512-particle chunks while the
+                                          forward's G2P runs their two 256-particle halves */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or

@@ -247,9 +247,11 @@ def test_autotune_picks_a_candidate_on_the_device():
     assert opts.pop("sort_every") in A.sort_candidates(6)
     assert opts.pop("reduce_parts") in (1, 2, 4)
     assert opts.pop("xgrad_stream") in (False, True)
+    assert opts.pop("g2p_split") <= opts["gather_particles_per_thread"]
     assert opts in A.candidates()
     assert res["warm_steps"] == min((sc.max_steps - 1) // 2, sc.max_steps - 1 - 12)
-    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1 + 2 + 1
+    n_split = len(A.stage_candidates("g2p_split", 6, opts)) - 1
+    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1 + 2 + 1 + n_split
     assert all(np.isfinite(t["ms"]) and t["ms"] > 0 for t in res["timings_ms"])
     A.clear_cache()
 
@@ -451,3 +453,26 @@ def test_position_gradient_stream_gives_the_same_gradient():
                 continue
             assert force_err(Fb, Fa) <= 1e-6, key
             assert grad_err(db, da) <= 1e-6, key
+
+
+def test_g2p_split_gives_the_same_result():
+    """The forward's G2P running each gather chunk as 2 or 4 workgroups (chunks of 2 or 4
+    particles per thread, as the adjoint P2G runs them) gives the one-workgroup state exactly and
+    the forces and gradient to the rounding of the FP64 stick-force atomics."""
+    sc = S.tiny_scene(16, 20, seed=8)
+    out = {}
+    with Engine([sc], ckpt_every=6) as e:
+        for gppt, split in ((2, 1), (2, 2), (4, 1), (4, 2), (4, 4)):
+            print(f"[gpu] gather particles per thread {gppt}, G2P split {split}", flush=True)
+            e.configure(gather_particles_per_thread=gppt, g2p_split=split)
+            e.reset()
+            e.forward()
+            F = e.forces()
+            st = e.read_liquid(0)
+            out[(gppt, split)] = (F.copy(), st, e.backward(np.ones_like(F)).copy())
+    Fa, sa, da = out[(2, 1)]
+    for key, (F, st, d) in out.items():
+        for k in ("x", "v", "C", "F"):
+            np.testing.assert_array_equal(st[k], sa[k], err_msg=str(key))
+        np.testing.assert_allclose(F, Fa, rtol=1e-10, atol=1e-10 * np.abs(Fa).max(), err_msg=str(key))
+        np.testing.assert_allclose(d, da, rtol=1e-10, atol=1e-10 * np.abs(da).max(), err_msg=str(key))
