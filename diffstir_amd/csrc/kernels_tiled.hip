@@ -361,28 +361,49 @@ __device__ __forceinline__ void load_neighbour_buckets(int* s_nc, int* s_cs, con
 // for local nodes 0..2, b and b+1 for local node 3; the node's index in the tile of bucket
 // b + d is local - 4d + 1.  The 24 (species, covering bucket) pairs are visited together:
 // round k loads the k-th chunk slot of every pair at once, so a round costs one memory latency.
-// This call takes rounds k0, k0 + kstep, ... (the parts of a split reduction).
+// This call takes rounds k0, k0 + kstep, ... (the parts of a split reduction).  kReread: read
+// each pair's table entries from LDS again in every round instead of holding 48 values in
+// registers across the rounds (the adjoint reduction: 126 instead of 143 registers, a fourth
+// wavefront per SIMD, about half the time on the 32-instance batch; the forward reduction's
+// other live values keep it at 3 wavefronts either way, so it keeps the registers).
+template <bool kReread>
 __device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __restrict__ staging, const int* s_nc,
                                                    const int* s_cs, int lx, int ly, int lz, int ncomp, int k0 = 0,
                                                    int kstep = 1)
 {
     const int ox = lx < 3 ? -1 : 0, oy = ly < 3 ? -1 : 0, oz = lz < 3 ? -1 : 0;
+    // pair q's bucket entry in the LDS tables and the node's index in its chunks' slots
+    auto pair = [&](int q, int& e) {
+        const int sp = q >> 3, dx = ox + ((q >> 2) & 1), dy = oy + ((q >> 1) & 1), dz = oz + (q & 1);
+        e = ((lx - 4 * dx + 1) * TS + (ly - 4 * dy + 1)) * TS + (lz - 4 * dz + 1);
+        return sp * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+    };
     int first[24], nc[24];
     int kmax = 0;
 #pragma unroll
     for (int q = 0; q < 24; q++) {
-        const int sp = q >> 3, dx = ox + ((q >> 2) & 1), dy = oy + ((q >> 1) & 1), dz = oz + (q & 1);
-        const int r = sp * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-        const int e = ((lx - 4 * dx + 1) * TS + (ly - 4 * dy + 1)) * TS + (lz - 4 * dz + 1);
+        int e;
+        const int r = pair(q, e);
         nc[q] = s_nc[r];
-        first[q] = s_cs[r] * TNODES + e;
+        if (!kReread) first[q] = s_cs[r] * TNODES + e;
         kmax = max(kmax, nc[q]);
     }
     for (int k = k0; k < kmax; k += kstep) {
         float4 v[24];
+        if (kReread) asm volatile("" ::: "memory");   // keeps the table reads inside the round
 #pragma unroll
-        for (int q = 0; q < 24; q++)
-            v[q] = k < nc[q] ? staging[first[q] + k * TNODES] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int q = 0; q < 24; q++) {
+            int n = nc[q], f = 0;
+            if (kReread) {
+                int e;
+                const int r = pair(q, e);
+                n = s_nc[r];
+                f = s_cs[r] * TNODES + e;
+            } else {
+                f = first[q];
+            }
+            v[q] = k < n ? staging[f + k * TNODES] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
 #pragma unroll
         for (int q = 0; q < 24; q++) {
             acc.x += v[q].x;
@@ -399,7 +420,7 @@ __device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __
 // others' partial sums in wavefront order.  Part 0 also takes the fall-back grid (read and
 // cleared) first.  Every thread of the workgroup must call this; the result is meaningful in
 // part 0 only.
-template <int NPART>
+template <int NPART, bool kReread>
 __device__ __forceinline__ float4 staged_node_sum(const float4* __restrict__ staging, const int* s_nc, const int* s_cs,
                                                   float4* __restrict__ fallback, bool fl, long gid, bool inb, int lx,
                                                   int ly, int lz, int ncomp)
@@ -412,7 +433,7 @@ __device__ __forceinline__ float4 staged_node_sum(const float4* __restrict__ sta
             acc = fallback[gid];
             fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        sum_covering_slots(acc, staging, s_nc, s_cs, lx, ly, lz, ncomp, part, NPART);
+        sum_covering_slots<kReread>(acc, staging, s_nc, s_cs, lx, ly, lz, ncomp, part, NPART);
     }
     if (NPART > 1) {
         if (part > 0) s_part[(part - 1) * 64 + node] = acc;
@@ -462,7 +483,7 @@ __global__ void __launch_bounds__(64 * NPART) k_grid_reduce(const float4* __rest
     const bool inb = gx < n && gy < n && gz < n;
     const long gid = (long)inst * nn + (inb ? node_index(n, gx, gy, gz) : 0);
     if (fl && threadIdx.x == 0) fbflag[blockIdx.x] = 0;
-    const float4 acc = staged_node_sum<NPART>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 4);
+    const float4 acc = staged_node_sum<NPART, false>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 4);
     if (!inb || threadIdx.x >= 64) return;
     if (pout) pout[gid] = acc;
     float v[3];
@@ -762,7 +783,7 @@ __global__ void __launch_bounds__(64 * NPART, 3) k_vbar_reduce(const float4* __r
     const bool inb = gx < n && gy < n && gz < n;
     const long gid = (long)inst * nn + (inb ? node_index(n, gx, gy, gz) : 0);
     if (fl && threadIdx.x == 0) fbflag[blockIdx.x] = 0;
-    const float4 acc = staged_node_sum<NPART>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 3);
+    const float4 acc = staged_node_sum<NPART, true>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 3);
     if (!inb || threadIdx.x >= 64) return;
     float v[3];
     NodeRec r;
