@@ -1,8 +1,8 @@
 // kernels_tiled.hip -- the substep and its adjoint on sorted particles with LDS grid tiles,
 // without floating-point atomics on the hot path.
 //
-// Particles are re-sorted at every checkpoint (every K substeps) into buckets of
-// (instance, species, 4x4x4-cell block).  Inside a bucket the order is round-robin over the
+// Particles are re-sorted at every checkpoint (every K substeps, or every R dividing K) into
+// buckets of (instance, species, 4x4x4-cell block).  Inside a bucket the order is round-robin over the
 // block's cells (the first particle of every occupied cell, then the second, ...), so the 64
 // lanes of a wavefront sit in different cells.  A bucket is cut into chunks of up to
 // 256 * PPT particles; one workgroup (4 wavefronts, PPT particles per thread) takes a chunk.
@@ -22,16 +22,18 @@
 //     memory barriers keep the compiler from reordering them);
 //   * the workgroup sums its four wave tiles and stores the sum in the chunk's staging slot
 //     (8 KB of plain coalesced stores);
-//   * a reduction kernel, one thread per grid node, adds the staging slots of the (at most
-//     eight per species) chunks whose tiles cover the node plus the fall-back grid, and applies
-//     the grid update there.  The sum order is fixed, so the grid is deterministic.
+//   * a reduction kernel, one thread per grid node (one to four wavefronts per 4x4x4 node
+//     block), adds the staging slots of the chunks whose tiles cover the node plus the
+//     fall-back grid, and applies the grid update there.  The sum order is fixed, so the grid
+//     is deterministic.
 //
 //   k_p2g_tiled       P2G of a chunk into its staging slot
 //   k_grid_reduce     staged momentum/mass -> grid update -> velocity (and momentum/mass)
 //   k_g2p_tiled       gather from a velocity tile, particle update, stick force
-//   k_g2p_adj_tiled   adjoint G2P: velocity tile from momentum/mass, dL/dv scatter to staging
+//   k_g2p_adj_tiled   adjoint G2P: dL/dv scatter to staging
 //   k_vbar_reduce     staged dL/dv -> adjoint grid update -> dL/d(momentum, mass)
-//   k_p2g_adj_tiled   adjoint P2G gather, FP64 parameter-gradient reduction
+//   k_p2g_adj_xgrad   adjoint P2G's position gradient through G2P's gather (optional, own stream)
+//   k_p2g_adj_tiled   adjoint P2G gathers and stress adjoint, FP64 parameter-gradient reduction
 // The P2G wave tiles hold one float4 (momentum, mass) per node in the staging slot's node order
 // (x 64, y 8, z 1), so a node's read-modify-write is one 16-byte LDS read and one write.  The
 // other LDS tiles are component planes with strides (x 68, y 8, z 1): for the 64 cells of a
