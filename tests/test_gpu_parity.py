@@ -476,3 +476,27 @@ def test_g2p_split_gives_the_same_result():
             np.testing.assert_array_equal(st[k], sa[k], err_msg=str(key))
         np.testing.assert_allclose(F, Fa, rtol=1e-10, atol=1e-10 * np.abs(Fa).max(), err_msg=str(key))
         np.testing.assert_allclose(d, da, rtol=1e-10, atol=1e-10 * np.abs(da).max(), err_msg=str(key))
+
+
+@pytest.mark.parametrize("opts", [
+    dict(sort_every=5, reduce_parts=4, xgrad_stream=True, g2p_split=2, gather_particles_per_thread=2),
+    dict(sort_every=7, reduce_parts=2, xgrad_stream=True, adjoint_p2g="split", scatter_chunks="packed"),
+    dict(reduce_parts=2, xgrad_stream=False, g2p_split=4, gather_particles_per_thread=4, particles_per_thread=1),
+])
+def test_later_stage_options_give_the_oracle_result(opts):
+    """Combinations of the autotuner's later-stage options (sort interval, reduction parts, the
+    position-gradient stream, G2P split) with the kernel options give the oracle's forces and
+    gradient on the golden n_grid 32 scene, launched and replayed as a hipGraph."""
+    sc, z = gold_scene("plain_n32_20")
+    for graph in (False, True):
+        print(f"[gpu] options {opts}, graph {graph}", flush=True)
+        with Engine([sc], ckpt_every=35 if opts.get("sort_every") == 7 else 10, max_steps=21, graph=graph) as e:
+            e.configure(**opts)
+            for rep in range(2 if graph else 1):
+                e.reset()
+                e.forward()
+                F = e.forces()[0]
+                _, g = compute_loss(F, z["target"], sc.cfg.loss_kind)
+                d = e.backward(g[None])[0]
+                assert force_err(F, z["forces"]) <= FORCE_TOL, (opts, graph)
+                assert grad_err(d, z["dl_dparams"]) <= GRAD_TOL, (opts, graph)
