@@ -36,8 +36,10 @@
 //   k_p2g_adj_tiled   adjoint P2G gathers and stress adjoint, FP64 parameter-gradient reduction
 // The P2G wave tiles hold one float4 (momentum, mass) per node in the staging slot's node order
 // (x 64, y 8, z 1), so a node's read-modify-write is one 16-byte LDS read and one write.  The
-// other LDS tiles are component planes with strides (x 68, y 8, z 1): for the 64 cells of a
-// block the node offsets of one stencil point fall in 32 distinct banks per half-wavefront.
+// other LDS tiles are component planes.  The adjoint G2P's wave tiles and the forward G2P's
+// velocity tile use strides (x 68, y 8, z 1): for the 64 cells of a block the node offsets of one
+// stencil point fall in 32 distinct banks per half-wavefront.  The adjoint P2G's tiles use
+// (x 112, y 12, z 1), whose offsets fall in 64 distinct banks over the whole wavefront.
 #include "ds_particle.h"
 
 namespace ds {
@@ -93,6 +95,16 @@ __device__ __forceinline__ bool scatter_slot(const GChunk& ch, const int* __rest
 }
 
 __device__ __forceinline__ int tile_id(int lx, int ly, int lz) { return lx * TSX + ly * TSY + lz; }
+
+// A component-plane layout: node (lx, ly, lz) at lx * XSTR + ly * YSTR + lz of each plane, the
+// planes `size` floats apart.
+template <int XSTR, int YSTR>
+struct Planes {
+    static constexpr int size = TS * XSTR;
+    __device__ static __forceinline__ int id(int lx, int ly, int lz) { return lx * XSTR + ly * YSTR + lz; }
+};
+using PlanesS = Planes<TSX, TSY>;   // scatter wave tiles, forward gathers
+using PlanesG = Planes<112, 12>;    // adjoint P2G gathers (1.4 KB more LDS per plane)
 
 __device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0, int z0)
 {
@@ -166,23 +178,26 @@ struct WaveScatter3 {
 // Gather accessors for a workgroup tile.  A particle whose stencil left the tile gathers from
 // global memory instead; the kernels branch once per particle between the two accessors, so
 // each inlined 27-node gather is a single straight-line form.
-struct TileV {
+template <class L>
+struct TileVT {
     const float* t;
     int x0, y0, z0;
     __device__ __forceinline__ float3 get(int i, int j, int k) const
     {
-        const int id = tile_id(i - x0, j - y0, k - z0);
-        return make_float3(t[id], t[TILE + id], t[2 * TILE + id]);
+        const int id = L::id(i - x0, j - y0, k - z0);
+        return make_float3(t[id], t[L::size + id], t[2 * L::size + id]);
     }
 };
+using TileV = TileVT<PlanesS>;
 
-struct TileP4 {
+template <class L>
+struct TileP4T {
     const float* t;
     int x0, y0, z0;
     __device__ __forceinline__ float4 get(int i, int j, int k) const
     {
-        const int id = tile_id(i - x0, j - y0, k - z0);
-        return make_float4(t[id], t[TILE + id], t[2 * TILE + id], t[3 * TILE + id]);
+        const int id = L::id(i - x0, j - y0, k - z0);
+        return make_float4(t[id], t[L::size + id], t[2 * L::size + id], t[3 * L::size + id]);
     }
 };
 
@@ -656,7 +671,8 @@ __global__ void __launch_bounds__(WG) k_liquid_j(const float* __restrict__ st, c
 
 // ---------------------------------------------------------------------------- adjoint
 
-// Velocity tile from the momentum/mass grid: the grid update at every node of the tile.
+// Velocity tile (layout L) from the momentum/mass grid: the grid update at every node of the tile.
+template <class L>
 __device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __restrict__ gi, const DevScene& sc,
                                                    int x0, int y0, int z0)
 {
@@ -669,10 +685,10 @@ __device__ __forceinline__ void load_velocity_tile(float* tv, const float4* __re
             NodeRec r;
             gridop_node(sc, gi[node_index(n, gx, gy, gz)], gx, gy, gz, v, r);
         }
-        const int id = tile_id(lx, ly, lz);
+        const int id = L::id(lx, ly, lz);
         tv[id] = v[0];
-        tv[TILE + id] = v[1];
-        tv[2 * TILE + id] = v[2];
+        tv[L::size + id] = v[1];
+        tv[2 * L::size + id] = v[2];
     }
 }
 
@@ -865,8 +881,8 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
     const int nchunks = min(*nchunks_p, maxchunks);
-    __shared__ float tq[4 * TILE];
-    __shared__ float tv[3 * TILE];
+    __shared__ float tq[4 * PlanesG::size];
+    __shared__ float tv[3 * PlanesG::size];
     __shared__ double red[WAVES][4];
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
@@ -880,14 +896,14 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
             float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) v = qi[node_index(n, gx, gy, gz)];
-            const int id = tile_id(lx, ly, lz);
+            const int id = PlanesG::id(lx, ly, lz);
             tq[id] = v.x;
-            tq[TILE + id] = v.y;
-            tq[2 * TILE + id] = v.z;
-            tq[3 * TILE + id] = v.w;
+            tq[PlanesG::size + id] = v.y;
+            tq[2 * PlanesG::size + id] = v.z;
+            tq[3 * PlanesG::size + id] = v.w;
         }
         const float4* gi = grid + (long)ch.inst * nn;
-        if (!kXp && ch.species == LIQUID) load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        if (!kXp && ch.species == LIQUID) load_velocity_tile<PlanesG>(tv, gi, sc, x0, y0, z0);
         __syncthreads();
         double tb[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
@@ -922,7 +938,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
 #pragma unroll
                     for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
                     if (it) {
-                        TileV gv{tv, x0, y0, z0};
+                        TileVT<PlanesG> gv{tv, x0, y0, z0};
                         g2p_adj_xgrad(sc, s, vt, Cb, gv, xp);
                     } else {
                         GatherVfromP gv{gi, sc};
@@ -942,7 +958,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
                 const float u[3] = {st[SV * LP + p], st[(SV + 1) * LP + p] + sc.gdt, st[(SV + 2) * LP + p]};
                 if (it) {
-                    TileP4 g{tq, x0, y0, z0};
+                    TileP4T<PlanesG> g{tq, x0, y0, z0};
                     p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
                 } else {
                     GatherPbar g{qi, n};
@@ -963,7 +979,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             } else {
                 float Co[9];
                 if (it) {
-                    TileP4 g{tq, x0, y0, z0};
+                    TileP4T<PlanesG> g{tq, x0, y0, z0};
                     p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
                 } else {
                     GatherPbar g{qi, n};
@@ -990,7 +1006,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_xgrad(const float* __restrict__ 
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
     const int nchunks = min(*nchunks_p, maxchunks);
-    __shared__ float tv[3 * TILE];
+    __shared__ float tv[3 * PlanesG::size];
     const long nn = (long)sc.n_grid * sc.n_grid * sc.n_grid;
     const long LP = P;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -998,7 +1014,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_xgrad(const float* __restrict__ 
         if (ch.species != LIQUID) continue;   // uniform across the workgroup
         const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
         const float4* gi = grid + (long)ch.inst * nn;
-        load_velocity_tile(tv, gi, sc, x0, y0, z0);
+        load_velocity_tile<PlanesG>(tv, gi, sc, x0, y0, z0);
         __syncthreads();
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
@@ -1018,7 +1034,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_xgrad(const float* __restrict__ 
 #pragma unroll
             for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
             if (stencil_in_tile(s, x0, y0, z0)) {
-                TileV gv{tv, x0, y0, z0};
+                TileVT<PlanesG> gv{tv, x0, y0, z0};
                 g2p_adj_xgrad(sc, s, vt, Cb, gv, g);
             } else {
                 GatherVfromP gv{gi, sc};
