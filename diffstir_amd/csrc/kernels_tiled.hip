@@ -39,7 +39,7 @@
 // other LDS tiles are component planes.  The adjoint G2P's wave tiles and the forward G2P's
 // velocity tile use strides (x 68, y 8, z 1): for the 64 cells of a block the node offsets of one
 // stencil point fall in 32 distinct banks per half-wavefront.  The adjoint P2G's tiles use
-// (x 112, y 12, z 1), whose offsets fall in 64 distinct banks over the whole wavefront.
+// (x 112, y 12, z 1), with which its tile reads take fewer LDS instructions (more of them pair).
 #include "ds_particle.h"
 
 namespace ds {
