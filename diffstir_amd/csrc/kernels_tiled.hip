@@ -148,7 +148,7 @@ __device__ __forceinline__ void mark_fallback_blocks(int* __restrict__ fbflag, i
 // Read-modify-write scatter into a wavefront's private tile (see the claim protocol above).
 // The barrier after each node keeps the next node's reads behind this node's writes: another
 // lane's stencil may reach the same node at a different offset.  WaveScatter4: one float4 per
-// node (TNODES per wave, staging order); WaveScatter3: three component planes.
+// node (TNODES per wave, staging order); WaveScatter21 (below): a float2 and a float plane.
 struct WaveScatter4 {
     float4* t;
     int x0, y0, z0;
@@ -161,16 +161,20 @@ struct WaveScatter4 {
     }
 };
 
-struct WaveScatter3 {
+// WaveScatter21: a float2 plane (first two components) followed by a float plane (the third), so
+// a node's read-modify-write is one 8-byte and one 4-byte LDS read and write.
+struct WaveScatter21 {
     float* t;
     int x0, y0, z0;
     __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c)
     {
-        float* q = t + tile_id(i - x0, j - y0, k - z0);
-        const float qa = q[0], qb = q[TILE], qc = q[2 * TILE];
-        q[0] = qa + a;
-        q[TILE] = qb + b;
-        q[2 * TILE] = qc + c;
+        const int id = tile_id(i - x0, j - y0, k - z0);
+        float2* q2 = reinterpret_cast<float2*>(t) + id;
+        float* q1 = t + 2 * TILE + id;
+        const float2 o = *q2;
+        const float oc = *q1;
+        *q2 = make_float2(o.x + a, o.y + b);
+        *q1 = oc + c;
         lds_order();
     }
 };
@@ -243,18 +247,22 @@ __device__ __forceinline__ void store_staging4(const float4* wt, float4* __restr
     }
 }
 
-// Sum the wave tiles (ncomp planes each, stride wstride floats) of the workgroup into the
-// chunk's staging slot: TNODES float4 in (lx, ly, lz) order, unused components 0.
-__device__ __forceinline__ void store_staging(const float* wt, int wstride, int ncomp, float4* __restrict__ slot)
+// Sum the WaveScatter21 tiles (3 * TILE floats each) of the workgroup into the chunk's staging slot:
+// TNODES float4 in (lx, ly, lz) order, the fourth component 0.
+__device__ __forceinline__ void store_staging21(const float* wt, float4* __restrict__ slot)
 {
     for (int e = threadIdx.x; e < TNODES; e += WG) {
         const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
         const int id = tile_id(lx, ly, lz);
-        float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float v[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int w = 0; w < WAVES; w++)
-            for (int c = 0; c < ncomp; c++) v[c] += wt[w * wstride + c * TILE + id];
-        slot[e] = make_float4(v[0], v[1], v[2], v[3]);
+        for (int w = 0; w < WAVES; w++) {
+            const float2 u = reinterpret_cast<const float2*>(wt + w * 3 * TILE)[id];
+            v[0] += u.x;
+            v[1] += u.y;
+            v[2] += wt[w * 3 * TILE + 2 * TILE + id];
+        }
+        slot[e] = make_float4(v[0], v[1], v[2], 0.0f);
     }
 }
 
@@ -733,7 +741,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
         float x[3], Cb[9], vt[3];
         if (valid) load(p, x, Cb, vt);
         zero_lds(wt, WAVES * 3 * TILE);
-        WaveScatter3 acc{wt + wave * 3 * TILE, x0, y0, z0};
+        WaveScatter21 acc{wt + wave * 3 * TILE, x0, y0, z0};
 #pragma unroll 1
         for (int r = 0; r < PPT; r++) {
             Stencil s;
@@ -770,7 +778,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
             valid = vn;
         }
         __syncthreads();
-        store_staging(wt, 3 * TILE, 3, staging + (long)c * TNODES);
+        store_staging21(wt, staging + (long)c * TNODES);
         __syncthreads();
     }
 }
