@@ -38,8 +38,9 @@
 // (x 64, y 8, z 1), so a node's read-modify-write is one 16-byte LDS read and one write.  The
 // other LDS tiles are component planes.  The adjoint G2P's wave tiles and the forward G2P's
 // velocity tile use strides (x 68, y 8, z 1): for the 64 cells of a block the node offsets of one
-// stencil point fall in 32 distinct banks per half-wavefront.  The adjoint P2G's tiles use
-// (x 112, y 12, z 1), with which its tile reads take fewer LDS instructions (more of them pair).
+// stencil point fall in 32 distinct banks per half-wavefront.  The adjoint P2G's velocity tiles
+// use (x 112, y 12, z 1), with which its tile reads take fewer LDS instructions (more of them
+// pair); its dL/d(momentum, mass) tile holds one float4 per node in staging order.
 #include "ds_particle.h"
 
 namespace ds {
@@ -104,7 +105,7 @@ struct Planes {
     __device__ static __forceinline__ int id(int lx, int ly, int lz) { return lx * XSTR + ly * YSTR + lz; }
 };
 using PlanesS = Planes<TSX, TSY>;   // scatter wave tiles, forward gathers
-using PlanesG = Planes<112, 12>;    // adjoint P2G gathers (1.4 KB more LDS per plane)
+using PlanesG = Planes<112, 12>;    // adjoint P2G velocity tiles (1.4 KB more LDS per plane)
 
 __device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0, int z0)
 {
@@ -194,14 +195,13 @@ struct TileVT {
 };
 using TileV = TileVT<PlanesS>;
 
-template <class L>
-struct TileP4T {
-    const float* t;
+// A float4 tile in staging node order (x 64, y 8, z 1): one 16-byte read per node.
+struct TileQ4 {
+    const float4* t;
     int x0, y0, z0;
     __device__ __forceinline__ float4 get(int i, int j, int k) const
     {
-        const int id = L::id(i - x0, j - y0, k - z0);
-        return make_float4(t[id], t[L::size + id], t[2 * L::size + id], t[3 * L::size + id]);
+        return t[((i - x0) * TS + (j - y0)) * TS + (k - z0)];
     }
 };
 
@@ -889,7 +889,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
     const int nchunks = min(*nchunks_p, maxchunks);
-    __shared__ float tq[4 * PlanesG::size];
+    __shared__ float4 tq[TNODES];
     __shared__ float tv[3 * PlanesG::size];
     __shared__ double red[WAVES][4];
     const int n = sc.n_grid;
@@ -904,11 +904,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
             float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) v = qi[node_index(n, gx, gy, gz)];
-            const int id = PlanesG::id(lx, ly, lz);
-            tq[id] = v.x;
-            tq[PlanesG::size + id] = v.y;
-            tq[2 * PlanesG::size + id] = v.z;
-            tq[3 * PlanesG::size + id] = v.w;
+            tq[e] = v;
         }
         const float4* gi = grid + (long)ch.inst * nn;
         if (!kXp && ch.species == LIQUID) load_velocity_tile<PlanesG>(tv, gi, sc, x0, y0, z0);
@@ -966,7 +962,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
                 const float u[3] = {st[SV * LP + p], st[(SV + 1) * LP + p] + sc.gdt, st[(SV + 2) * LP + p]};
                 if (it) {
-                    TileP4T<PlanesG> g{tq, x0, y0, z0};
+                    TileQ4 g{tq, x0, y0, z0};
                     p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
                 } else {
                     GatherPbar g{qi, n};
@@ -987,7 +983,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             } else {
                 float Co[9];
                 if (it) {
-                    TileP4T<PlanesG> g{tq, x0, y0, z0};
+                    TileQ4 g{tq, x0, y0, z0};
                     p2g_adj_rigid(sc, species_mass(sc, ch.species), s, g, Co);
                 } else {
                     GatherPbar g{qi, n};
