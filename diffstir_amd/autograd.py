@@ -7,7 +7,9 @@
 The forward runs ds_reset + ds_forward over the whole trajectory for every instance of the
 engine; the backward hands dL/dforces to the library's checkpointed adjoint.  Tensors live on
 the host (the force series and the parameters are a few numbers per instance); the particle
-state never leaves the GPU.  This replaces the reference's ti.ad.Tape over forward(s) calls
+state never leaves the GPU.  An engine holds one recorded rollout: a second apply on the same
+engine, or new parameters, before loss.backward() makes that backward raise instead of
+returning the gradient of another rollout.  This replaces the reference's ti.ad.Tape over forward(s) calls
 (src/gradient_descent_module.py:103-107) for torch-side losses and optimisers.
 """
 from __future__ import annotations
@@ -27,10 +29,14 @@ class StirRollout(torch.autograd.Function):
         engine.forward()
         F = engine.forces()
         ctx.engine = engine
+        ctx.generation = engine.generation
         return torch.from_numpy(F).to(dtype=torch.float64, device=params.device)
 
     @staticmethod
     def backward(ctx, grad_forces: torch.Tensor):
+        if ctx.engine.generation != ctx.generation:
+            raise RuntimeError("StirRollout.backward: the engine was reset or given new parameters after this "
+                               "rollout was recorded; its adjoint would belong to another rollout")
         g = grad_forces.detach().to("cpu", torch.float64).numpy()
         d = ctx.engine.backward(np.ascontiguousarray(g))
         return torch.from_numpy(d).to(dtype=torch.float64, device=grad_forces.device), None

@@ -63,6 +63,9 @@ struct ds_ctx {
     std::vector<double> params_h; // [B][4]
     int step = 0;
     bool reset_done = false;
+    // the parameters or a trajectory changed after the rollout was recorded: the adjoint's
+    // recompute would replay a different rollout than the recorded forces (cleared by ds_reset)
+    bool tape_stale = false;
 
     // common
     int* d_meta = nullptr;
@@ -1318,6 +1321,7 @@ int ds_set_trajectory(ds_ctx* c, int32_t inst, const float* traj)
     if (!c) return DS_EINVAL;
     if (inst < 0 || inst >= c->B || !traj) return fail(c, DS_EINVAL, "bad instance or NULL trajectory");
     std::memcpy(c->traj_h.data() + (size_t)inst * c->T * 3, traj, sizeof(float) * 3 * c->T);
+    if (c->step > 0) c->tape_stale = true;
     HIPCHK(c, hipMemcpyAsync(c->d_traj + (size_t)inst * c->T * 3, c->traj_h.data() + (size_t)inst * c->T * 3,
                              sizeof(float) * 3 * c->T, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1330,6 +1334,7 @@ int ds_set_params(ds_ctx* c, const double* params)
     if (!params) return fail(c, DS_EINVAL, "NULL params");
     for (int i = 0; i < 4 * c->B; i++)
         if (!std::isfinite(params[i])) return fail(c, DS_EINVAL, "non-finite parameter");
+    if (c->step > 0 && std::memcmp(c->params_h.data(), params, sizeof(double) * 4 * c->B) != 0) c->tape_stale = true;
     std::memcpy(c->params_h.data(), params, sizeof(double) * 4 * c->B);
     HIPCHK(c, hipMemcpyAsync(c->d_params, c->params_h.data(), sizeof(double) * 4 * c->B, hipMemcpyHostToDevice,
                              c->stream));
@@ -1353,6 +1358,7 @@ int ds_reset(ds_ctx* c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->step = 0;
     c->reset_done = true;
+    c->tape_stale = false;
     return DS_OK;
 }
 
@@ -1435,6 +1441,9 @@ int ds_backward(ds_ctx* c, const double* dl_dforces, double* dl_dparams)
     if (!c) return DS_EINVAL;
     if (!dl_dforces || !dl_dparams) return fail(c, DS_EINVAL, "NULL argument");
     if (!c->reset_done || c->step < 1) return fail(c, DS_ESTATE, "ds_backward needs a recorded forward rollout");
+    if (c->tape_stale)
+        return fail(c, DS_ESTATE, "the parameters or a trajectory changed after the rollout was recorded: "
+                                  "ds_reset and ds_forward again before ds_backward");
     // dL/d(dv sum at step s) = m_stick/dt * dL/dF(s)
     std::vector<double> dvb((size_t)c->B * c->T * 3);
     const double scale = c->desc.m_stick / c->desc.dt;

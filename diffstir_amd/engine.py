@@ -56,6 +56,9 @@ class Engine:
                  gather_particles_per_thread: int = 1, adjoint_p2g: str = "fused", scatter_chunks: str = "rows",
                  graph: bool = False, overlap: bool = True):
         self.lib = _lib.load()
+        # bumped by every call that invalidates a recorded rollout (reset, new parameters, new
+        # particles or trajectory): a consumer holding a rollout checks it before the adjoint
+        self.generation = 0
         scenes = list(scenes)
         if not scenes:
             raise ValueError("need at least one scene")
@@ -101,6 +104,7 @@ class Engine:
         xs = _f32(s.x_stick, (-1, 3))
         xc = _f32(s.x_container, (-1, 3))
         self._check(self.lib.ds_set_particles(self.ctx, b, _p32(xl), _p32(xs), _p32(xc)))
+        self.generation += 1
         tr = np.asarray(s.traj[:self.T], dtype=np.float32)
         if tr.shape[0] < self.T:
             tr = np.concatenate([tr, np.repeat(tr[-1:], self.T - tr.shape[0], axis=0)])
@@ -201,11 +205,14 @@ class Engine:
     def set_params(self, params):
         p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(self.B, 4))
         self._check(self.lib.ds_set_params(self.ctx, _p64(p)))
+        if not np.array_equal(p, getattr(self, "params", None)):
+            self.generation += 1
         self.params = p.copy()
 
     def reset(self):
         """initialize_ti_field (simulation_functions.py:301-319)."""
         self._check(self.lib.ds_reset(self.ctx))
+        self.generation += 1
 
     def forward(self, nsteps: Optional[int] = None):
         """Advance nsteps substeps (default: to the end of the trajectory)."""

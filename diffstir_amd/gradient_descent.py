@@ -67,42 +67,6 @@ def single_parameter_fitting1(parameter: float, parameter_grad: float, lr: float
     return parameter, lr
 
 
-def single_parameter_fitting(parameter: float, parameter_grad: float, lr: float, lower_bound: float,
-                             upper_bound: float, epochs: int, epoch_index: int) -> Tuple[float, float]:
-    """gdm:44-82 (defined but unused by the reference driver): a cosine-annealed step between the
-    rates that would reach either bound, NaN-safe."""
-    period = epochs - epoch_index
-    base = 3
-    level_to_top = magnitude_level((upper_bound - parameter) / period)
-    level_to_bottom = magnitude_level((lower_bound - parameter) / period)
-    if not np.isnan(parameter_grad) and abs(parameter_grad) > 1e-31:
-        delta = lr * parameter_grad
-        lr_to_top = lr * pow(base, level_to_top - magnitude_level(delta))
-        lr_to_bottom = lr * pow(base, level_to_bottom - magnitude_level(delta))
-        max_lr = max(lr_to_top, lr_to_bottom)
-        min_lr = min(lr_to_top, lr_to_bottom)
-        lr = min_lr + 0.5 * (max_lr - min_lr) * (1 + np.cos(epoch_index / epochs))
-        parameter_update = parameter - lr * parameter_grad
-        if lower_bound < parameter_update < upper_bound:
-            parameter = parameter_update
-        else:
-            new_magnitude = magnitude_level(delta)
-            if parameter_update <= lower_bound:
-                new_magnitude = level_to_top
-            if parameter_update >= upper_bound:
-                new_magnitude = level_to_bottom
-            lr *= pow(base, new_magnitude - magnitude_level(delta))
-            delta = lr * parameter_grad
-            parameter_update = parameter - delta
-            for _ in range(4096):
-                if not (parameter_update > upper_bound or parameter_update < lower_bound):
-                    break
-                lr *= 0.5
-                parameter_update = parameter - lr * parameter_grad
-            parameter = parameter_update
-    return float(parameter), float(lr)
-
-
 def parameters_fitting(sim: Simulator, epoch_setting: int, lower_bound: Sequence[float],
                        upper_bound: Sequence[float], verbose: bool = False,
                        n_fit: Optional[int] = None, first_clip: bool = True) -> Tuple[List[float], dict]:

@@ -130,7 +130,9 @@ int ds_set_particles(ds_ctx* ctx, int32_t inst, const float* x_liquid, const flo
                      const float* x_container);
 /* Stick trajectory of one instance, [max_steps][3] (operator_kinematic_seq, initialization.py:151). */
 int ds_set_trajectory(ds_ctx* ctx, int32_t inst, const float* traj);
-/* Constitutive parameters, [n_inst][4] (constitutive_parameters, initialization.py:134). */
+/* Constitutive parameters, [n_inst][4] (constitutive_parameters, initialization.py:134).  New values
+ * after a recorded rollout (as a new trajectory, ds_set_trajectory) make ds_backward return
+ * DS_ESTATE until the next ds_reset: the adjoint recomputes its segments with the current values. */
 int ds_set_params(ds_ctx* ctx, const double* params);
 
 /* Step-0 state from the initial positions (v = 0, F = I, C = 0), forces zeroed, step counter

@@ -5,10 +5,19 @@
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The product
  * library (diffstir_amd/csrc) never links, calls or falls back to it.
  *
- * Compiled twice from this one source (oracle/Makefile): REAL=double is the FP64
- * oracle, REAL=float is an FP32 restatement with the reference's structure, used only
- * to size FP32 rounding drift.  Parameters, masses, the loss and every accumulation of
- * the parameter gradient are double in both builds.
+ * Compiled three times from this one source (oracle/Makefile): REAL=double is the FP64
+ * oracle; REAL=float is an FP32 restatement, used to size FP32 rounding drift; REAL=float
+ * with ORACLE_ATOMIC is the reference's execution structure (one parallel loop over the
+ * particles of each species with atomic adds into a dense grid, as Taichi runs sf:104-196 on
+ * ti.cpu), used only as the CPU baseline of the benchmark.  Parameters, masses, the loss and
+ * every accumulation of the parameter gradient are double in every build.
+ *
+ * Threads (OpenMP).  Without ORACLE_ATOMIC the results do not depend on the thread count:
+ * gathers and per-node work are independent; scatters run over groups of particles with one
+ * stencil-base column (bx, bz), the nine residue classes (bx mod 3, bz mod 3) one after the
+ * other, so groups running at once never write the same node and every node receives its
+ * contributions in an order fixed by the data; the stick-force and parameter-gradient sums
+ * are formed per particle and added in particle order.
  *
  * Reference semantics followed (file:line in the DiffStir reference,
  * sf = src/simulator/simulation_functions.py, init = src/simulator/initialization.py):
@@ -34,15 +43,28 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef REAL
 #define REAL double
 #endif
 
-#ifdef ORACLE_F32
+#if defined(ORACLE_ATOMIC)
+#define OR_SYM(name) ora32_##name
+#elif defined(ORACLE_F32)
 #define OR_SYM(name) or32_##name
 #else
 #define OR_SYM(name) or_##name
+#endif
+
+/* grid accumulation: atomic in the reference-structured build, plain (race-free by the
+ * column colouring) otherwise */
+#ifdef ORACLE_ATOMIC
+#define GRID_ADD(dst, val) do { REAL _v = (val); _Pragma("omp atomic") (dst) += _v; } while (0)
+#else
+#define GRID_ADD(dst, val) ((dst) += (val))
 #endif
 
 typedef struct {
@@ -238,9 +260,126 @@ static void scatter(const or_scene* sc, REAL* P, const int b[3], const REAL fx[3
                 REAL W = w[0][i] * w[1][j] * w[2][k];
                 REAL* g = P + 4 * nidx(sc, b[0] + i, b[1] + j, b[2] + k);
                 for (int a = 0; a < 3; a++)
-                    g[a] += W * (m * u[a] + A[a * 3 + 0] * dp[0] + A[a * 3 + 1] * dp[1] + A[a * 3 + 2] * dp[2]);
-                g[3] += W * m;
+                    GRID_ADD(g[a], W * (m * u[a] + A[a * 3 + 0] * dp[0] + A[a * 3 + 1] * dp[1] + A[a * 3 + 2] * dp[2]));
+                GRID_ADD(g[3], W * m);
             }
+}
+
+/* ------------------------------------------------------------------ parallel loops */
+
+/* Particles of one species grouped by stencil-base column g = bx * n + bz (a counting sort,
+ * index order within a group).  Checks every position first: returns the error of the lowest
+ * failing particle index, as a sequential loop would. */
+typedef struct {
+    int np;
+    int* start;   /* [n*n + 1] */
+    int* list;    /* [np] */
+} colgroups;
+
+static void colgroups_free(colgroups* g)
+{
+    free(g->start);
+    free(g->list);
+    g->start = g->list = NULL;
+}
+
+static int colgroups_build(const or_scene* sc, int np, const REAL* X, colgroups* g)
+{
+    int n = sc->n_grid;
+    long ng = (long)n * n;
+    g->np = np;
+    g->start = (int*)calloc(ng + 1, sizeof(int));
+    g->list = (int*)malloc(sizeof(int) * (np > 0 ? np : 1));
+    int* key = (int*)malloc(sizeof(int) * (np > 0 ? np : 1));
+    if (!g->start || !g->list || !key) { free(key); colgroups_free(g); return OR_NOMEM; }
+    int first_bad = np;
+    #pragma omp parallel for reduction(min : first_bad)
+    for (int p = 0; p < np; p++) {
+        int b[3];
+        REAL fx[3], w[3][3], dw[3][3];
+        if (or_bspline(sc, X + 3 * p, b, fx, w, dw)) { if (p < first_bad) first_bad = p; key[p] = 0; }
+        else key[p] = b[0] * n + b[2];
+    }
+    if (first_bad < np) {
+        int b[3];
+        REAL fx[3], w[3][3], dw[3][3];
+        int rc = or_bspline(sc, X + 3 * first_bad, b, fx, w, dw);
+        free(key);
+        colgroups_free(g);
+        return rc;
+    }
+    for (int p = 0; p < np; p++) g->start[key[p] + 1]++;
+    for (long i = 0; i < ng; i++) g->start[i + 1] += g->start[i];
+    int* fill = (int*)malloc(sizeof(int) * (ng > 0 ? ng : 1));
+    if (!fill) { free(key); colgroups_free(g); return OR_NOMEM; }
+    memcpy(fill, g->start, sizeof(int) * ng);
+    for (int p = 0; p < np; p++) g->list[fill[key[p]]++] = p;
+    free(fill);
+    free(key);
+    return OR_OK;
+}
+
+typedef void (*particle_fn)(void* ctx, int p);
+
+/* Run fn on every particle of a scatter.  ORACLE_ATOMIC: one parallel loop over the particles
+ * (fn adds atomically).  Otherwise: the nine column colours in turn, the groups of one colour
+ * in parallel (a group writes node columns [bx, bx+2] x [bz, bz+2] only). */
+static void run_scatter(const or_scene* sc, const colgroups* g, particle_fn fn, void* ctx)
+{
+#ifdef ORACLE_ATOMIC
+    (void)sc;
+    #pragma omp parallel for schedule(static)
+    for (int p = 0; p < g->np; p++) fn(ctx, p);
+#else
+    int n = sc->n_grid;
+    for (int c = 0; c < 9; c++) {
+        int cx = c / 3, cz = c % 3;
+        int nx = (n - cx + 2) / 3, nz = (n - cz + 2) / 3;
+        #pragma omp parallel for schedule(dynamic, 1) collapse(2)
+        for (int ix = 0; ix < nx; ix++)
+            for (int iz = 0; iz < nz; iz++) {
+                long gi = (long)(cx + 3 * ix) * n + (cz + 3 * iz);
+                for (int t = g->start[gi]; t < g->start[gi + 1]; t++) fn(ctx, g->list[t]);
+            }
+    }
+#endif
+}
+
+/* P2G of the liquid: sf:104-163 */
+typedef struct {
+    const or_scene* sc;
+    const double* th;
+    const or_state* s;
+    REAL* P;
+    REAL* Fnew;
+} p2g_liq_ctx;
+
+static void p2g_liq_one(void* vc, int p)
+{
+    p2g_liq_ctx* c = (p2g_liq_ctx*)vc;
+    const or_state* s = c->s;
+    liq_pre q;
+    liquid_pre(c->sc, c->th, s->lx + 3 * p, s->lv + 3 * p, s->lC + 9 * p, s->lF + 9 * p, &q);
+    if (c->Fnew) memcpy(c->Fnew + 9 * p, q.Fn, sizeof(REAL) * 9);
+    scatter(c->sc, c->P, q.b, q.fx, q.w, (REAL)c->sc->m_liq, q.u, q.A);
+}
+
+/* P2G of the stick and container: sf:165-196 (affine = m C, no stress, no gravity) */
+typedef struct {
+    const or_scene* sc;
+    const REAL *X, *V, *Cm;
+    REAL m;
+    REAL* P;
+} p2g_rigid_ctx;
+
+static void p2g_rigid_one(void* vc, int p)
+{
+    p2g_rigid_ctx* c = (p2g_rigid_ctx*)vc;
+    int b[3];
+    REAL fx[3], w[3][3], dw[3][3], A[9];
+    or_bspline(c->sc, c->X + 3 * p, b, fx, w, dw);
+    for (int i = 0; i < 9; i++) A[i] = c->m * c->Cm[9 * p + i];
+    scatter(c->sc, c->P, b, fx, w, c->m, c->V + 3 * p, A);
 }
 
 /* P2G for all species into P (n^3 x 4: momentum xyz, mass).  Writes F_new for liquid. */
@@ -248,27 +387,19 @@ static int p2g_all(const or_scene* sc, const double* th, const or_state* s, REAL
 {
     long nn = (long)sc->n_grid * sc->n_grid * sc->n_grid;
     memset(P, 0, sizeof(REAL) * 4 * nn);
-    liq_pre q;
-    for (int p = 0; p < s->nl; p++) {
-        int rc = liquid_pre(sc, th, s->lx + 3 * p, s->lv + 3 * p, s->lC + 9 * p, s->lF + 9 * p, &q);
-        if (rc) return rc;
-        if (Fnew) memcpy(Fnew + 9 * p, q.Fn, sizeof(REAL) * 9);
-        scatter(sc, P, q.b, q.fx, q.w, (REAL)sc->m_liq, q.u, q.A);
-    }
+    colgroups g;
+    int rc = colgroups_build(sc, s->nl, s->lx, &g);
+    if (rc) return rc;
+    p2g_liq_ctx lc = {sc, th, s, P, Fnew};
+    run_scatter(sc, &g, p2g_liq_one, &lc);
+    colgroups_free(&g);
     for (int sp = 0; sp < 2; sp++) {
-        int n = sp ? s->nc : s->ns;
-        const REAL* X = sp ? s->cx : s->sx;
-        const REAL* V = sp ? s->cv : s->sv;
-        const REAL* Cm = sp ? s->cC : s->sC;
-        REAL m = (REAL)(sp ? sc->m_cont : sc->m_stick);
-        for (int p = 0; p < n; p++) {
-            int b[3];
-            REAL fx[3], w[3][3], dw[3][3], A[9];
-            int rc = or_bspline(sc, X + 3 * p, b, fx, w, dw);
-            if (rc) return rc;
-            for (int i = 0; i < 9; i++) A[i] = m * Cm[9 * p + i];
-            scatter(sc, P, b, fx, w, m, V + 3 * p, A);
-        }
+        p2g_rigid_ctx rc2 = {sc, sp ? s->cx : s->sx, sp ? s->cv : s->sv, sp ? s->cC : s->sC,
+                             (REAL)(sp ? sc->m_cont : sc->m_stick), P};
+        rc = colgroups_build(sc, sp ? s->nc : s->ns, rc2.X, &g);
+        if (rc) return rc;
+        run_scatter(sc, &g, p2g_rigid_one, &rc2);
+        colgroups_free(&g);
     }
     return OR_OK;
 }
@@ -315,6 +446,7 @@ static void gridop_node(const or_scene* sc, const REAL* Pn, int i, int j, int k,
 static void gridop_all(const or_scene* sc, const REAL* P, REAL* Vout)
 {
     int n = sc->n_grid;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++)
             for (int k = 0; k < n; k++) {
@@ -342,16 +474,42 @@ static void gather(const or_scene* sc, const REAL* Vout, const int b[3], const R
             }
 }
 
+/* First failing particle index (np if none) of a species, and its error. */
+static int first_bad_particle(const or_scene* sc, int np, const REAL* X, int* rc)
+{
+    int first = np;
+    #pragma omp parallel for reduction(min : first)
+    for (int p = 0; p < np; p++) {
+        int b[3];
+        REAL fx[3], w[3][3], dw[3][3];
+        if (or_bspline(sc, X + 3 * p, b, fx, w, dw) && p < first) first = p;
+    }
+    *rc = OR_OK;
+    if (first < np) {
+        int b[3];
+        REAL fx[3], w[3][3], dw[3][3];
+        *rc = or_bspline(sc, X + 3 * first, b, fx, w, dw);
+    }
+    return first;
+}
+
 /* G2P for all species from state `s` into `o`; dvsum receives sum_p dv over stick particles. */
 static int g2p_all(const or_scene* sc, const REAL* Vout, const or_state* s, or_state* o,
                    const double* traj_f, const double* traj_f1, double* dvsum)
 {
     REAL dt = (REAL)sc->dt;
-    int b[3];
-    REAL fx[3], w[3][3], dw[3][3], nv[3], nC[9];
+    int rc;
+    first_bad_particle(sc, s->nl, s->lx, &rc);
+    if (rc) return rc;
+    first_bad_particle(sc, s->ns, s->sx, &rc);
+    if (rc) return rc;
+    first_bad_particle(sc, s->nc, s->cx, &rc);
+    if (rc) return rc;
+    #pragma omp parallel for schedule(static)
     for (int p = 0; p < s->nl; p++) {
-        int rc = or_bspline(sc, s->lx + 3 * p, b, fx, w, dw);
-        if (rc) return rc;
+        int b[3];
+        REAL fx[3], w[3][3], dw[3][3], nv[3], nC[9];
+        or_bspline(sc, s->lx + 3 * p, b, fx, w, dw);
         gather(sc, Vout, b, fx, w, nv, nC);
         for (int a = 0; a < 3; a++) {
             o->lv[3 * p + a] = nv[a];
@@ -361,21 +519,30 @@ static int g2p_all(const or_scene* sc, const REAL* Vout, const or_state* s, or_s
     }
     REAL vk[3];
     for (int a = 0; a < 3; a++) vk[a] = (REAL)((REAL)traj_f1[a] - (REAL)traj_f[a]) / dt;
-    double acc[3] = {0, 0, 0};
+    double* dvp = (double*)malloc(sizeof(double) * 3 * (s->ns > 0 ? s->ns : 1));
+    if (!dvp) return OR_NOMEM;
+    #pragma omp parallel for schedule(static)
     for (int p = 0; p < s->ns; p++) {
-        int rc = or_bspline(sc, s->sx + 3 * p, b, fx, w, dw);
-        if (rc) return rc;
+        int b[3];
+        REAL fx[3], w[3][3], dw[3][3], nv[3], nC[9];
+        or_bspline(sc, s->sx + 3 * p, b, fx, w, dw);
         gather(sc, Vout, b, fx, w, nv, nC);
         for (int a = 0; a < 3; a++) {
-            acc[a] += (double)(vk[a] - nv[a]);
+            dvp[3 * p + a] = (double)(vk[a] - nv[a]);
             o->sv[3 * p + a] = vk[a];
             o->sx[3 * p + a] = s->sx[3 * p + a] + vk[a] * dt;
         }
         memcpy(o->sC + 9 * p, nC, sizeof(REAL) * 9);
     }
+    double acc[3] = {0, 0, 0};
+    for (int p = 0; p < s->ns; p++)
+        for (int a = 0; a < 3; a++) acc[a] += dvp[3 * p + a];
+    free(dvp);
+    #pragma omp parallel for schedule(static)
     for (int p = 0; p < s->nc; p++) {
-        int rc = or_bspline(sc, s->cx + 3 * p, b, fx, w, dw);
-        if (rc) return rc;
+        int b[3];
+        REAL fx[3], w[3][3], dw[3][3], nv[3], nC[9];
+        or_bspline(sc, s->cx + 3 * p, b, fx, w, dw);
         gather(sc, Vout, b, fx, w, nv, nC);
         for (int a = 0; a < 3; a++) {
             o->cv[3 * p + a] = 0;
@@ -412,6 +579,171 @@ int OR_SYM(step)(const or_scene* sc, const double* th, const or_state* in, or_st
 
 /* ------------------------------------------------------------------ adjoint */
 
+/* G2P adjoint of one liquid particle: scatter into Vb, partial x-bar (sf:230-248 reversed) */
+typedef struct {
+    const or_scene* sc;
+    const or_state* s;
+    const or_adj* ob;
+    const REAL* V;
+    REAL* Vb;
+    REAL* xb;
+} g2p_adj_ctx;
+
+static void g2p_adj_liq_one(void* vc, int p)
+{
+    g2p_adj_ctx* c = (g2p_adj_ctx*)vc;
+    const or_scene* sc = c->sc;
+    REAL dt = (REAL)sc->dt, inv_dx = (REAL)sc->inv_dx;
+    REAL c4 = (REAL)4 * inv_dx;
+    int b[3];
+    REAL fx[3], w[3][3], dw[3][3];
+    or_bspline(sc, c->s->lx + 3 * p, b, fx, w, dw);
+    const REAL* Cb = c->ob->lC + 9 * p;
+    REAL vt[3], fxb[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) vt[a] = c->ob->lv[3 * p + a] + dt * c->ob->lx[3 * p + a];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                REAL d[3] = {(REAL)i - fx[0], (REAL)j - fx[1], (REAL)k - fx[2]};
+                REAL W = w[0][i] * w[1][j] * w[2][k];
+                REAL dW[3] = {dw[0][i] * w[1][j] * w[2][k], w[0][i] * dw[1][j] * w[2][k],
+                              w[0][i] * w[1][j] * dw[2][k]};
+                long id = nidx(sc, b[0] + i, b[1] + j, b[2] + k);
+                const REAL* g = c->V + 3 * id;
+                REAL cd[3];
+                for (int a = 0; a < 3; a++)
+                    cd[a] = Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2];
+                for (int a = 0; a < 3; a++) GRID_ADD(c->Vb[3 * id + a], W * (vt[a] + c4 * cd[a]));
+                REAL Wb = g[0] * vt[0] + g[1] * vt[1] + g[2] * vt[2]
+                        + c4 * (g[0] * cd[0] + g[1] * cd[1] + g[2] * cd[2]);
+                for (int cc = 0; cc < 3; cc++) {
+                    REAL ctg = Cb[0 * 3 + cc] * g[0] + Cb[1 * 3 + cc] * g[1] + Cb[2 * 3 + cc] * g[2];
+                    fxb[cc] += Wb * dW[cc] - c4 * W * ctg;
+                }
+            }
+    for (int a = 0; a < 3; a++) c->xb[3 * p + a] = c->ob->lx[3 * p + a] + fxb[a] * inv_dx;
+}
+
+/* G2P adjoint of one stick (dv = v_kin - sum W v_grid, sf:250-270) or container particle */
+typedef struct {
+    const or_scene* sc;
+    const REAL* X;
+    const REAL* CB;
+    const double* dvbar;   /* NULL for the container */
+    REAL* Vb;
+} g2p_adj_rigid_ctx;
+
+static void g2p_adj_rigid_one(void* vc, int p)
+{
+    g2p_adj_rigid_ctx* c = (g2p_adj_rigid_ctx*)vc;
+    const or_scene* sc = c->sc;
+    REAL c4 = (REAL)4 * (REAL)sc->inv_dx;
+    int b[3];
+    REAL fx[3], w[3][3], dw[3][3];
+    or_bspline(sc, c->X + 3 * p, b, fx, w, dw);
+    const REAL* Cb = c->CB + 9 * p;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                REAL d[3] = {(REAL)i - fx[0], (REAL)j - fx[1], (REAL)k - fx[2]};
+                REAL W = w[0][i] * w[1][j] * w[2][k];
+                long id = nidx(sc, b[0] + i, b[1] + j, b[2] + k);
+                for (int a = 0; a < 3; a++) {
+                    REAL cd = Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2];
+                    REAL t = c4 * cd;
+                    if (c->dvbar) t -= (REAL)c->dvbar[a];
+                    GRID_ADD(c->Vb[3 * id + a], W * t);
+                }
+            }
+}
+
+/* P2G adjoint of one liquid particle: gather Pbar, stress and rheology adjoint; writes the
+ * particle's share of dL/dparams into thp[4] */
+static void p2g_adj_liq_one(const or_scene* sc, const double* th, const or_state* s, const or_adj* ob,
+                            const REAL* Pb, const REAL* xb, or_adj* ib, int p, double* thp)
+{
+    REAL dx = (REAL)sc->dx, inv_dx = (REAL)sc->inv_dx, dt = (REAL)sc->dt;
+    REAL m = (REAL)sc->m_liq;
+    REAL alpha = (REAL)(-sc->dt * sc->vol_liq * 4.0 * sc->inv_dx * sc->inv_dx);
+    double kk = bulk_k(sc, th);
+    liq_pre q;
+    const REAL* C = s->lC + 9 * p;
+    const REAL* F = s->lF + 9 * p;
+    liquid_pre(sc, th, s->lx + 3 * p, s->lv + 3 * p, C, F, &q);
+    REAL ub[3] = {0, 0, 0}, Ab[9] = {0}, fxb[3] = {0, 0, 0};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                REAL dp[3] = {((REAL)i - q.fx[0]) * dx, ((REAL)j - q.fx[1]) * dx, ((REAL)k - q.fx[2]) * dx};
+                REAL W = q.w[0][i] * q.w[1][j] * q.w[2][k];
+                REAL dW[3] = {q.dw[0][i] * q.w[1][j] * q.w[2][k], q.w[0][i] * q.dw[1][j] * q.w[2][k],
+                              q.w[0][i] * q.w[1][j] * q.dw[2][k]};
+                long id = nidx(sc, q.b[0] + i, q.b[1] + j, q.b[2] + k);
+                const REAL* g = Pb + 4 * id;
+                REAL h = g[3];
+                REAL mom[3];
+                for (int a = 0; a < 3; a++)
+                    mom[a] = m * q.u[a] + q.A[a * 3] * dp[0] + q.A[a * 3 + 1] * dp[1] + q.A[a * 3 + 2] * dp[2];
+                for (int a = 0; a < 3; a++) {
+                    ub[a] += W * m * g[a];
+                    for (int c = 0; c < 3; c++) Ab[a * 3 + c] += W * g[a] * dp[c];
+                }
+                REAL Wb = g[0] * mom[0] + g[1] * mom[1] + g[2] * mom[2] + h * m;
+                for (int c = 0; c < 3; c++) {
+                    REAL atg = q.A[0 * 3 + c] * g[0] + q.A[1 * 3 + c] * g[1] + q.A[2 * 3 + c] * g[2];
+                    fxb[c] += Wb * dW[c] - dx * W * atg;
+                }
+            }
+    REAL* oxb = ib->lx + 3 * p;
+    REAL* ovb = ib->lv + 3 * p;
+    REAL* oCb = ib->lC + 9 * p;
+    REAL* oFb = ib->lF + 9 * p;
+    REAL Fnb[9];
+    for (int a = 0; a < 9; a++) Fnb[a] = ob->lF[9 * p + a];   /* adjoint of F[s+1] = F_tmp */
+    for (int a = 0; a < 3; a++) oxb[a] = xb[3 * p + a] + fxb[a] * inv_dx;
+    for (int a = 0; a < 3; a++) ovb[a] = ub[a];
+    REAL Cb[9], Sb[9];
+    for (int a = 0; a < 9; a++) { Cb[a] = m * Ab[a]; Sb[a] = alpha * Ab[a]; }
+    /* S = eta sym(C) + I (k/2)(J - 1/J - 1) J */
+    double etab = 0;
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++)
+            etab += (double)Sb[a * 3 + c] * 0.5 * ((double)C[a * 3 + c] + (double)C[c * 3 + a]);
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++)
+            Cb[a * 3 + c] += q.eta * (REAL)0.5 * (Sb[a * 3 + c] + Sb[c * 3 + a]);
+    double trS = (double)Sb[0] + (double)Sb[4] + (double)Sb[8];
+    double J = (double)q.J;
+    double Jb = trS * (kk / 2.0) * (2.0 * J - 1.0);
+    double deta_dg, deta_dth[4];
+    visc(sc->model, th, (double)q.gamma, &deta_dg, deta_dth);
+    for (int a = 0; a < 4; a++) thp[a] = etab * deta_dth[a];
+    if (sc->k_from_param) thp[3] += trS * 0.5 * (J - 1.0 / J - 1.0) * J;
+    double gb = etab * deta_dg;
+    double n2 = 0;
+    for (int a = 0; a < 9; a++) n2 += (double)C[a] * (double)C[a];
+    if (n2 > 0) {
+        double nrm = sqrt(n2);
+        for (int a = 0; a < 9; a++) Cb[a] += (REAL)(gb * (double)C[a] / nrm);
+    }
+    REAL K[9];
+    cof3(q.Fn, K);
+    for (int a = 0; a < 9; a++) Fnb[a] += (REAL)Jb * K[a];
+    /* Fn = (I + dt C) F */
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++) {
+            REAL sF = 0, sC2 = 0;
+            for (int k = 0; k < 3; k++) {
+                REAL Mka = (k == a ? (REAL)1 : (REAL)0) + dt * C[k * 3 + a];
+                sF += Mka * Fnb[k * 3 + c];           /* (I+dtC)^T Fnb */
+                sC2 += Fnb[a * 3 + k] * F[c * 3 + k]; /* Fnb F^T */
+            }
+            oFb[a * 3 + c] = sF;
+            Cb[a * 3 + c] += dt * sC2;
+        }
+    memcpy(oCb, Cb, sizeof(REAL) * 9);
+}
+
 /* Adjoint of one substep.  Given the state at s and the adjoints of the state at s+1
  * (`ob`) and of the stick's summed dv at s+1 (`dvbar`, the same vector for every stick
  * particle), writes the adjoints of the state at s into `ib` and adds dL/dparams into
@@ -424,78 +756,40 @@ int OR_SYM(step_adjoint)(const or_scene* sc, const double* th, const or_state* s
     (void)traj_f; (void)traj_f1;
     long nn = grid_nodes(sc);
     int n = sc->n_grid;
+    int nl1 = s->nl > 0 ? s->nl : 1;
     REAL* P = (REAL*)malloc(sizeof(REAL) * 4 * nn);
     REAL* V = (REAL*)malloc(sizeof(REAL) * 3 * nn);
     REAL* Vb = (REAL*)calloc(3 * nn, sizeof(REAL));
     REAL* Pb = (REAL*)malloc(sizeof(REAL) * 4 * nn);
-    REAL* xb = (REAL*)malloc(sizeof(REAL) * 3 * (s->nl > 0 ? s->nl : 1));
+    REAL* xb = (REAL*)malloc(sizeof(REAL) * 3 * nl1);
+    double* thp = (double*)malloc(sizeof(double) * 4 * nl1);
+    colgroups g = {0, NULL, NULL};
     int rc = OR_OK;
-    if (!P || !V || !Vb || !Pb || !xb) { rc = OR_NOMEM; goto done; }
+    if (!P || !V || !Vb || !Pb || !xb || !thp) { rc = OR_NOMEM; goto done; }
 
     /* recompute the grid of this step (sf:353-358) */
     rc = p2g_all(sc, th, s, P, NULL);
     if (rc) goto done;
     gridop_all(sc, P, V);
 
-    REAL dx = (REAL)sc->dx, inv_dx = (REAL)sc->inv_dx, dt = (REAL)sc->dt;
-    REAL c4 = (REAL)4 * inv_dx;
-    int b[3];
-    REAL fx[3], w[3][3], dw[3][3];
-
     /* --- G2P adjoint: scatter into Vb, partial x-bar for liquid */
-    for (int p = 0; p < s->nl; p++) {
-        rc = or_bspline(sc, s->lx + 3 * p, b, fx, w, dw);
+    {
+        g2p_adj_ctx c = {sc, s, ob, V, Vb, xb};
+        rc = colgroups_build(sc, s->nl, s->lx, &g);
         if (rc) goto done;
-        const REAL* Cb = ob->lC + 9 * p;
-        REAL vt[3], fxb[3] = {0, 0, 0};
-        for (int a = 0; a < 3; a++) vt[a] = ob->lv[3 * p + a] + dt * ob->lx[3 * p + a];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                for (int k = 0; k < 3; k++) {
-                    REAL d[3] = {(REAL)i - fx[0], (REAL)j - fx[1], (REAL)k - fx[2]};
-                    REAL W = w[0][i] * w[1][j] * w[2][k];
-                    REAL dW[3] = {dw[0][i] * w[1][j] * w[2][k], w[0][i] * dw[1][j] * w[2][k],
-                                  w[0][i] * w[1][j] * dw[2][k]};
-                    long id = nidx(sc, b[0] + i, b[1] + j, b[2] + k);
-                    const REAL* g = V + 3 * id;
-                    REAL cd[3];
-                    for (int a = 0; a < 3; a++)
-                        cd[a] = Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2];
-                    for (int a = 0; a < 3; a++) Vb[3 * id + a] += W * (vt[a] + c4 * cd[a]);
-                    REAL Wb = g[0] * vt[0] + g[1] * vt[1] + g[2] * vt[2]
-                            + c4 * (g[0] * cd[0] + g[1] * cd[1] + g[2] * cd[2]);
-                    for (int c = 0; c < 3; c++) {
-                        REAL ctg = Cb[0 * 3 + c] * g[0] + Cb[1 * 3 + c] * g[1] + Cb[2 * 3 + c] * g[2];
-                        fxb[c] += Wb * dW[c] - c4 * W * ctg;
-                    }
-                }
-        for (int a = 0; a < 3; a++) xb[3 * p + a] = ob->lx[3 * p + a] + fxb[a] * inv_dx;
+        run_scatter(sc, &g, g2p_adj_liq_one, &c);
+        colgroups_free(&g);
     }
     for (int sp = 0; sp < 2; sp++) {
-        int np = sp ? s->nc : s->ns;
-        const REAL* X = sp ? s->cx : s->sx;
-        const REAL* CB = sp ? ob->cC : ob->sC;
-        for (int p = 0; p < np; p++) {
-            rc = or_bspline(sc, X + 3 * p, b, fx, w, dw);
-            if (rc) goto done;
-            const REAL* Cb = CB + 9 * p;
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++)
-                    for (int k = 0; k < 3; k++) {
-                        REAL d[3] = {(REAL)i - fx[0], (REAL)j - fx[1], (REAL)k - fx[2]};
-                        REAL W = w[0][i] * w[1][j] * w[2][k];
-                        long id = nidx(sc, b[0] + i, b[1] + j, b[2] + k);
-                        for (int a = 0; a < 3; a++) {
-                            REAL cd = Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2];
-                            REAL t = c4 * cd;
-                            if (!sp) t -= (REAL)dvbar[a];   /* dv = v_kin - sum W v_grid */
-                            Vb[3 * id + a] += W * t;
-                        }
-                    }
-        }
+        g2p_adj_rigid_ctx c = {sc, sp ? s->cx : s->sx, sp ? ob->cC : ob->sC, sp ? NULL : dvbar, Vb};
+        rc = colgroups_build(sc, sp ? s->nc : s->ns, c.X, &g);
+        if (rc) goto done;
+        run_scatter(sc, &g, g2p_adj_rigid_one, &c);
+        colgroups_free(&g);
     }
 
     /* --- grid-op adjoint: Vb -> (Pbar momentum, Pbar mass) */
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++)
             for (int k = 0; k < n; k++) {
@@ -503,126 +797,48 @@ int OR_SYM(step_adjoint)(const or_scene* sc, const double* th, const or_state* s
                 node_rec r;
                 REAL vo[3];
                 gridop_node(sc, P + 4 * id, i, j, k, vo, &r);
-                REAL g[3] = {Vb[3 * id], Vb[3 * id + 1], Vb[3 * id + 2]};
-                for (int a = 0; a < 3; a++) if (r.mask[a]) g[a] = 0;
+                REAL gg[3] = {Vb[3 * id], Vb[3 * id + 1], Vb[3 * id + 2]};
+                for (int a = 0; a < 3; a++) if (r.mask[a]) gg[a] = 0;
                 if (r.wall) {
-                    REAL gn = g[0] * r.nrm[0] + g[1] * r.nrm[1] + g[2] * r.nrm[2];
-                    for (int a = 0; a < 3; a++) g[a] -= r.nrm[a] * gn;
+                    REAL gn = gg[0] * r.nrm[0] + gg[1] * r.nrm[1] + gg[2] * r.nrm[2];
+                    for (int a = 0; a < 3; a++) gg[a] -= r.nrm[a] * gn;
                 }
-                for (int a = 0; a < 3; a++) Pb[4 * id + a] = r.inv_m * g[a];
-                Pb[4 * id + 3] = -r.inv_m * (g[0] * r.v0[0] + g[1] * r.v0[1] + g[2] * r.v0[2]);
+                for (int a = 0; a < 3; a++) Pb[4 * id + a] = r.inv_m * gg[a];
+                Pb[4 * id + 3] = -r.inv_m * (gg[0] * r.v0[0] + gg[1] * r.v0[1] + gg[2] * r.v0[2]);
             }
 
-    /* --- P2G adjoint: gather Pbar */
-    {
-        REAL m = (REAL)sc->m_liq;
-        REAL alpha = (REAL)(-sc->dt * sc->vol_liq * 4.0 * sc->inv_dx * sc->inv_dx);
-        double kk = bulk_k(sc, th);
-        liq_pre q;
-        for (int p = 0; p < s->nl; p++) {
-            const REAL* C = s->lC + 9 * p;
-            const REAL* F = s->lF + 9 * p;
-            rc = liquid_pre(sc, th, s->lx + 3 * p, s->lv + 3 * p, C, F, &q);
-            if (rc) goto done;
-            REAL ub[3] = {0, 0, 0}, Ab[9] = {0}, fxb[3] = {0, 0, 0};
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++)
-                    for (int k = 0; k < 3; k++) {
-                        REAL dp[3] = {((REAL)i - q.fx[0]) * dx, ((REAL)j - q.fx[1]) * dx, ((REAL)k - q.fx[2]) * dx};
-                        REAL W = q.w[0][i] * q.w[1][j] * q.w[2][k];
-                        REAL dW[3] = {q.dw[0][i] * q.w[1][j] * q.w[2][k], q.w[0][i] * q.dw[1][j] * q.w[2][k],
-                                      q.w[0][i] * q.w[1][j] * q.dw[2][k]};
-                        long id = nidx(sc, q.b[0] + i, q.b[1] + j, q.b[2] + k);
-                        const REAL* g = Pb + 4 * id;
-                        REAL h = g[3];
-                        REAL mom[3];
-                        for (int a = 0; a < 3; a++)
-                            mom[a] = m * q.u[a] + q.A[a * 3] * dp[0] + q.A[a * 3 + 1] * dp[1] + q.A[a * 3 + 2] * dp[2];
-                        for (int a = 0; a < 3; a++) {
-                            ub[a] += W * m * g[a];
-                            for (int c = 0; c < 3; c++) Ab[a * 3 + c] += W * g[a] * dp[c];
-                        }
-                        REAL Wb = g[0] * mom[0] + g[1] * mom[1] + g[2] * mom[2] + h * m;
-                        for (int c = 0; c < 3; c++) {
-                            REAL atg = q.A[0 * 3 + c] * g[0] + q.A[1 * 3 + c] * g[1] + q.A[2 * 3 + c] * g[2];
-                            fxb[c] += Wb * dW[c] - dx * W * atg;
-                        }
-                    }
-            REAL* oxb = ib->lx + 3 * p;
-            REAL* ovb = ib->lv + 3 * p;
-            REAL* oCb = ib->lC + 9 * p;
-            REAL* oFb = ib->lF + 9 * p;
-            REAL Fnb[9];
-            for (int a = 0; a < 9; a++) Fnb[a] = ob->lF[9 * p + a];   /* adjoint of F[s+1] = F_tmp */
-            for (int a = 0; a < 3; a++) oxb[a] = xb[3 * p + a] + fxb[a] * inv_dx;
-            for (int a = 0; a < 3; a++) ovb[a] = ub[a];
-            REAL Cb[9], Sb[9];
-            for (int a = 0; a < 9; a++) { Cb[a] = m * Ab[a]; Sb[a] = alpha * Ab[a]; }
-            /* S = eta sym(C) + I (k/2)(J - 1/J - 1) J */
-            double etab = 0;
-            for (int a = 0; a < 3; a++)
-                for (int c = 0; c < 3; c++)
-                    etab += (double)Sb[a * 3 + c] * 0.5 * ((double)C[a * 3 + c] + (double)C[c * 3 + a]);
-            for (int a = 0; a < 3; a++)
-                for (int c = 0; c < 3; c++)
-                    Cb[a * 3 + c] += q.eta * (REAL)0.5 * (Sb[a * 3 + c] + Sb[c * 3 + a]);
-            double trS = (double)Sb[0] + (double)Sb[4] + (double)Sb[8];
-            double J = (double)q.J;
-            double Jb = trS * (kk / 2.0) * (2.0 * J - 1.0);
-            if (sc->k_from_param) th_bar[3] += trS * 0.5 * (J - 1.0 / J - 1.0) * J;
-            double deta_dg, deta_dth[4];
-            visc(sc->model, th, (double)q.gamma, &deta_dg, deta_dth);
-            for (int a = 0; a < 4; a++) th_bar[a] += etab * deta_dth[a];
-            double gb = etab * deta_dg;
-            double nC = (double)q.gamma - 1e-6;
-            double n2 = 0;
-            for (int a = 0; a < 9; a++) n2 += (double)C[a] * (double)C[a];
-            if (n2 > 0) {
-                double nrm = sqrt(n2);
-                (void)nC;
-                for (int a = 0; a < 9; a++) Cb[a] += (REAL)(gb * (double)C[a] / nrm);
-            }
-            REAL K[9];
-            cof3(q.Fn, K);
-            for (int a = 0; a < 9; a++) Fnb[a] += (REAL)Jb * K[a];
-            /* Fn = (I + dt C) F */
-            for (int a = 0; a < 3; a++)
-                for (int c = 0; c < 3; c++) {
-                    REAL sF = 0, sC2 = 0;
-                    for (int k = 0; k < 3; k++) {
-                        REAL Mka = (k == a ? (REAL)1 : (REAL)0) + dt * C[k * 3 + a];
-                        sF += Mka * Fnb[k * 3 + c];           /* (I+dtC)^T Fnb */
-                        sC2 += Fnb[a * 3 + k] * F[c * 3 + k]; /* Fnb F^T */
-                    }
-                    oFb[a * 3 + c] = sF;
-                    Cb[a * 3 + c] += dt * sC2;
-                }
-            memcpy(oCb, Cb, sizeof(REAL) * 9);
-        }
-    }
+    /* --- P2G adjoint: gather Pbar (positions were checked by the scatter's grouping) */
+    #pragma omp parallel for schedule(static)
+    for (int p = 0; p < s->nl; p++) p2g_adj_liq_one(sc, th, s, ob, Pb, xb, ib, p, thp + 4 * p);
+    for (int p = 0; p < s->nl; p++)
+        for (int a = 0; a < 4; a++) th_bar[a] += thp[4 * p + a];
     for (int sp = 0; sp < 2; sp++) {
         int np = sp ? s->nc : s->ns;
         const REAL* X = sp ? s->cx : s->sx;
         REAL* CB = sp ? ib->cC : ib->sC;
         REAL m = (REAL)(sp ? sc->m_cont : sc->m_stick);
+        REAL dx = (REAL)sc->dx;
+        #pragma omp parallel for schedule(static)
         for (int p = 0; p < np; p++) {
-            rc = or_bspline(sc, X + 3 * p, b, fx, w, dw);
-            if (rc) goto done;
+            int b[3];
+            REAL fx[3], w[3][3], dw[3][3];
+            or_bspline(sc, X + 3 * p, b, fx, w, dw);
             REAL Cb[9] = {0};
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++)
                     for (int k = 0; k < 3; k++) {
                         REAL dp[3] = {((REAL)i - fx[0]) * dx, ((REAL)j - fx[1]) * dx, ((REAL)k - fx[2]) * dx};
                         REAL W = w[0][i] * w[1][j] * w[2][k];
-                        const REAL* g = Pb + 4 * nidx(sc, b[0] + i, b[1] + j, b[2] + k);
+                        const REAL* gp = Pb + 4 * nidx(sc, b[0] + i, b[1] + j, b[2] + k);
                         for (int a = 0; a < 3; a++)
-                            for (int c = 0; c < 3; c++) Cb[a * 3 + c] += W * g[a] * dp[c];
+                            for (int c = 0; c < 3; c++) Cb[a * 3 + c] += W * gp[a] * dp[c];
                     }
             for (int a = 0; a < 9; a++) CB[9 * p + a] = m * Cb[a];
         }
     }
 done:
-    free(P); free(V); free(Vb); free(Pb); free(xb);
+    colgroups_free(&g);
+    free(P); free(V); free(Vb); free(Pb); free(xb); free(thp);
     return rc;
 }
 
@@ -790,4 +1006,17 @@ int OR_SYM(gather)(const or_scene* sc, const REAL* Vout, int n, const REAL* x, R
         gather(sc, Vout, b, fx, w, nv + 3 * p, nC + 9 * p);
     }
     return OR_OK;
+}
+
+/* Threads of the OpenMP loops for the calls that follow (n <= 0: leave the current setting);
+ * returns the number the next parallel loop will use (1 without OpenMP). */
+int OR_SYM(set_threads)(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

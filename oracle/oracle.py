@@ -16,7 +16,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_BUILD = os.path.join(_HERE, "_build")
+# ORACLE_BUILD selects another build directory, e.g. oracle/_build/asan from `make asan`
+_BUILD = os.environ.get("ORACLE_BUILD") or os.path.join(_HERE, "_build")
 
 
 class OrScene(C.Structure):
@@ -49,17 +50,27 @@ def build():
 
 
 class Oracle:
-    """One precision of the oracle: f64 (the oracle) or f32 (drift sizing)."""
+    """One build of the oracle: f64 (the oracle), f32 (FP32 drift sizing) or f32at (FP32 with the
+    reference's execution structure: per-particle parallel loops with atomic grid adds; the
+    benchmark's CPU baseline)."""
+
+    PREFIX = {"f64": "or_", "f32": "or32_", "f32at": "ora32_"}
 
     def __init__(self, precision: str = "f64"):
+        if precision not in self.PREFIX:
+            raise ValueError(f"precision must be one of {list(self.PREFIX)}")
         path = os.path.join(_BUILD, f"liboracle_{precision}.so")
         if not os.path.exists(path):
             build()
         self.lib = C.CDLL(path)
+        self.precision = precision
         self.real = C.c_double if precision == "f64" else C.c_float
         self.np_real = np.float64 if precision == "f64" else np.float32
         self.State, self.Adj = _state_struct(self.real)
-        pre = "or_" if precision == "f64" else "or32_"
+        pre = self.PREFIX[precision]
+        self._set_threads = getattr(self.lib, pre + "set_threads")
+        self._set_threads.restype = C.c_int
+        self._set_threads.argtypes = [C.c_int]
         self._step = getattr(self.lib, pre + "step")
         self._step_adj = getattr(self.lib, pre + "step_adjoint")
         self._rollout = getattr(self.lib, pre + "rollout")
@@ -71,6 +82,10 @@ class Oracle:
         self._eta.restype = C.c_double
         self._eta.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_double)]
+
+    def set_threads(self, n: int = 0) -> int:
+        """OpenMP threads of the following calls (0: keep); returns the count in use."""
+        return int(self._set_threads(int(n)))
 
     # ------------------------------------------------------------------ marshalling
     @staticmethod

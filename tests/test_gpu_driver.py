@@ -77,10 +77,14 @@ def test_force_replay_exports_reference_files(tmp_path):
     sp = np.load(os.path.join(tmp_path, "stick_positions.npy"))
     assert fx.dtype == np.float32 and fx.shape == (41,) and fz.shape == (41,)
     assert fx[0] == 0 and fz[0] == 0 and fx[-1] == 0 and fz[-1] == 0
-    assert sp.shape == (39, 3)
+    # 40 * dt -> 39 substeps fill frames 0..39 of the 41-frame field; fcam:56 drops frame 40
+    assert out["steps"] == 39
+    assert sp.shape == (40, 3)
     # the stick moves rigidly with the trajectory (sf:266-269)
-    np.testing.assert_allclose(sp - sp[0], sc.traj[:39] - sc.traj[0], atol=2e-6)
-    assert out["liquid_point_clouds"].shape == (40, sc.n_liquid, 3)
+    np.testing.assert_allclose(sp - sp[0], sc.traj[:40] - sc.traj[0], atol=2e-6)
+    pc = out["liquid_point_clouds"]
+    assert pc.shape == (41, sc.n_liquid, 3)
+    assert np.all(pc[40] == 0) and np.all(pc[39] != 0)
     o = Oracle("f64")
     Fo = o.rollout(sc, nsteps=39)
     assert force_err(np.stack([fx[:40], np.zeros(40), fz[:40]], 1), np.stack([Fo[:, 0], 0 * Fo[:, 0], Fo[:, 2]], 1)) <= 1e-4
@@ -107,6 +111,37 @@ def test_autograd_function_matches_engine_backward():
         e.forward()
         d = e.backward(g)
     np.testing.assert_allclose(p.grad.numpy(), d, rtol=1e-6)
+
+
+def test_stale_rollout_adjoint_is_an_error():
+    """An engine records one rollout.  Its adjoint after a second rollout (StirRollout applied
+    twice before loss.backward()) or after new parameters raises instead of returning the
+    gradient of another rollout; ds_backward itself refuses with DS_ESTATE after ds_set_params."""
+    torch = pytest.importorskip("torch")
+    from diffstir_amd._lib import DiffStirError
+    from diffstir_amd.autograd import StirRollout
+    sc = S.tiny_scene(16, 8, seed=2)
+    print("[gpu] stale rollout adjoint", flush=True)
+    with Engine([sc], ckpt_every=4) as e:
+        p1 = torch.tensor(sc.params[None], dtype=torch.float64, requires_grad=True)
+        F1 = StirRollout.apply(p1, e)
+        p2 = torch.tensor(sc.params[None] * 1.1, dtype=torch.float64, requires_grad=True)
+        F2 = StirRollout.apply(p2, e)
+        with pytest.raises(RuntimeError, match="another rollout"):
+            F1.sum().backward()
+        F2.sum().backward()                       # the latest rollout's adjoint still works
+        assert np.all(np.isfinite(p2.grad.numpy()))
+        g = np.zeros((1, e.T, 3))
+        g[0, 1:-1, 0] = 1.0
+        e.set_params(sc.params[None] * 1.2)
+        with pytest.raises(DiffStirError) as ei:
+            e.backward(g)
+        assert ei.value.code == 4                 # DS_ESTATE
+        e.reset()
+        e.forward()
+        assert np.all(np.isfinite(e.backward(g)))
+        e.set_params(sc.params[None] * 1.2)       # unchanged values keep the tape valid
+        assert np.all(np.isfinite(e.backward(g)))
 
 
 def test_batched_identification_reduces_the_loss():

@@ -95,6 +95,107 @@ def test_p2g_stress_is_internal_force(o64):
     assert radial > 0
 
 
+def _one_particle_affine(o64, cfg, params, F, Cm, xp=(0.503, 0.4981, 0.5117)):
+    """P2G of one liquid particle (v = 0, no gravity, no other species) and the affine matrix A
+    read back from the grid: sum_i w_i (x_i - x_p)(x_i - x_p)^T = dx^2/4 I for quadratic
+    B-splines (per axis sum w (i - fx)^2 = 1/4, sum w (i - fx) = 0), and the node values are
+    w_i (m v + A (x_i - x_p)), so 4/dx^2 * sum_i P_i (x_i - x_p)^T = A exactly, and the mass
+    moment sum_i m_i (x_i - x_p) is 0.  The base is int(x/dx - 0.5) (sf:108)."""
+    cfg = dataclasses.replace(cfg, gravity=0.0)
+    n, dx = cfg.n_grid, 1.0 / cfg.n_grid
+    st = {"lx": np.array([xp], np.float64), "lv": np.zeros((1, 3)), "lC": np.asarray(Cm, np.float64).reshape(1, 9),
+          "lF": np.asarray(F, np.float64).reshape(1, 9), "sx": np.zeros((0, 3)), "sv": np.zeros((0, 3)),
+          "sC": np.zeros((0, 9)), "cx": np.zeros((0, 3)), "cv": np.zeros((0, 3)), "cC": np.zeros((0, 9))}
+    P = o64.p2g(cfg, params, st).reshape(-1, 4)
+    idx = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    d = idx * dx - np.asarray(xp)
+    hit = P[:, 3] != 0
+    assert hit.sum() == 27
+    np.testing.assert_allclose(P[:, 3].sum(), cfg.m_liq, rtol=1e-13)
+    assert np.abs((P[hit, 3:4] * d[hit]).sum(0)).max() < 1e-12 * cfg.m_liq
+    return 4.0 / dx ** 2 * P[hit, :3].T @ d[hit]
+
+
+def _stress_cases():
+    """(name, scene config, params, k, eta(gamma) closed form) for the reference scene (k = 20000
+    hard-coded, sf:112, Carreau with n = 1: eta = eta0 for every gamma, sf:124) and a synthetic
+    workload scene (k = params[3], Herschel-Bulkley with n = 1: eta = tau_y / gamma + K,
+    sf:139-146)."""
+    ref = S.reference_scene(ref_traj(), seed=0, frames=3, params=(3.0, 0.2, 1.0, 5.0), model=0)
+    syn = S.workload("plain_n32")[0]
+    return [("reference_carreau", ref.cfg, np.array([3.0, 0.2, 1.0, 5.0]), 20000.0, lambda g: 3.0),
+            ("synthetic_hb", syn.cfg, np.array([2.0, 7.0, 1.0, 1000.0]), 1000.0, lambda g: 7.0 / g + 2.0)]
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_stress_assembly_closed_forms(o64, case):
+    """sf:148-151, read back from the grid through the second-moment identity above, on inputs
+    whose stress has a closed form (alpha = -dt V 4/dx^2, p(J) = (k/2)(J - 1/J - 1) J):
+
+      F = I, C = 0:                J = 1, p = -k/2, A = alpha p I = dt V 2k/dx^2 I
+      F = diag(2, 1, 1), C = 0:    J = 2, p = +k/2
+      F = I, C = s e0 e1^T:        J = det(I + dt C) = 1, gamma = |s| + 1e-6,
+                                   A = alpha (eta(gamma) (C + C^T)/2 - k/2 I) + m C
+
+    Each fails if the '- 1' of p is dropped (p(1) = 0), if the trailing J is dropped (p(2) = k/4),
+    if the 1/2 of sym(C) is dropped (A[1,0] doubles), or if 4/dx is used for 4/dx^2 (A scales by
+    dx)."""
+    name, cfg, th, k, eta = _stress_cases()[case]
+    dx = 1.0 / cfg.n_grid
+    alpha = -cfg.dt * cfg.vol_liq * 4.0 / dx ** 2
+    m = cfg.m_liq
+    eye = np.eye(3)
+    A = _one_particle_affine(o64, cfg, th, eye, np.zeros((3, 3)))
+    tol = 1e-12 * abs(alpha * k)
+    np.testing.assert_allclose(A, cfg.dt * cfg.vol_liq * 2.0 * k / dx ** 2 * eye, rtol=1e-10, atol=tol)
+    np.testing.assert_allclose(A, alpha * (-k / 2) * eye, rtol=1e-10, atol=tol)
+    A = _one_particle_affine(o64, cfg, th, np.diag([2.0, 1.0, 1.0]), np.zeros((3, 3)))
+    np.testing.assert_allclose(A, alpha * (k / 2) * eye, rtol=1e-10, atol=tol)
+    for s in (0.75, -3.0):
+        Cm = np.zeros((3, 3))
+        Cm[0, 1] = s
+        assert np.linalg.det(eye + cfg.dt * Cm) == 1.0
+        g = abs(s) + 1e-6
+        A = _one_particle_affine(o64, cfg, th, eye, Cm)
+        want = alpha * (eta(g) * 0.5 * (Cm + Cm.T) - k / 2 * eye) + m * Cm
+        np.testing.assert_allclose(A, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+        np.testing.assert_allclose(A[1, 0], alpha * eta(g) * s / 2, rtol=1e-10)
+
+
+_STRESS_MUTANTS = {
+    "pressure without the -1": ("(J - ((REAL)1 / J) - (REAL)1) * J", "(J - ((REAL)1 / J)) * J"),
+    "pressure without the trailing J": ("(J - ((REAL)1 / J) - (REAL)1) * J", "(J - ((REAL)1 / J) - (REAL)1)"),
+    "sym(C) without the 1/2": ("q->eta * (REAL)0.5 * (C[i * 3 + j] + C[j * 3 + i])",
+                               "q->eta * (C[i * 3 + j] + C[j * 3 + i])"),
+    "4/dx for 4/dx^2": ("4.0 * sc->inv_dx * sc->inv_dx", "4.0 * sc->inv_dx"),
+}
+
+
+@pytest.mark.parametrize("mutant", sorted(_STRESS_MUTANTS))
+def test_stress_pins_catch_planted_mistakes(tmp_path, mutant):
+    """Plant each plausible misreading of sf:148-151 in a copy of the oracle, build it, and check
+    that the closed forms above reject it for both scenes."""
+    import ctypes as C
+    import os
+    import subprocess
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                            "mpm_oracle.c")).read()
+    old, new = _STRESS_MUTANTS[mutant]
+    # the forward's stress assembly is liquid_pre(); plant the mistake there only
+    a, b = src.index("static int liquid_pre("), src.index("static void scatter(")
+    body = src[a:b]
+    assert body.count(old) == 1, f"oracle source changed: update the {mutant!r} mutant"
+    c_file = tmp_path / "mutant.c"
+    c_file.write_text(src[:a] + body.replace(old, new) + src[b:])
+    so = tmp_path / "libmutant.so"
+    subprocess.check_call(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-shared", "-o", str(so), str(c_file), "-lm"])
+    o = Oracle("f64")
+    o._p2g = C.CDLL(str(so)).or_p2g
+    for case in range(2):
+        with pytest.raises(AssertionError):
+            test_stress_assembly_closed_forms(o, case)
+
+
 # ----------------------------------------------------------------------------- rheology
 
 @pytest.mark.parametrize("g", [1e-3, 0.5, 3.0, 40.0])
