@@ -257,6 +257,8 @@ DevScene make_dev_scene(const ds_scene_desc& d)
     s.alpha = (float)(-d.dt * d.vol_liq * 4.0 * inv_dx * inv_dx);
     s.c4 = (float)(4.0 * inv_dx);
     s.k_bulk = d.k_bulk;
+    s.tile_win = TS - 3;
+    s.count_fallback = 0;
     return s;
 }
 
@@ -1176,14 +1178,14 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
     ALLOC(c->d_params, (size_t)c->B * 4);
     ALLOC(c->d_thbar, (size_t)c->B * 4);
     ALLOC(c->d_traj, (size_t)c->B * c->T * 3);
-    ALLOC(c->d_flag, 1);
+    ALLOC(c->d_flag, 4);   // [0] failure bits, [2..3] the 64-bit fall-back counter
 #undef ALLOC
     hipError_t e = hipStreamCreate(&c->stream);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess)
         e = hipMemcpy(c->d_groups, c->groups.data(), sizeof(Group) * c->groups.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(c->d_flag, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->d_flag, 0, 4 * sizeof(unsigned));
     if (e != hipSuccess) {
         std::string m = hipGetErrorString(e);
         free_all(c);
@@ -1284,6 +1286,18 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         if (value != 0 && value != 1) return fail(c, DS_EINVAL, "DS_OPT_OVERLAP takes 0 or 1");
         if (c->overlap != (value == 1)) drop_graphs(c);
         c->overlap = value == 1;
+        return DS_OK;
+    }
+    if (option == DS_OPT_TILE_WINDOW) {
+        if (value < 0 || value > TS - 3) return fail(c, DS_EINVAL, "DS_OPT_TILE_WINDOW takes 0..5");
+        if (c->dsc.tile_win != value) drop_graphs(c);   // the scene constants are kernel arguments
+        c->dsc.tile_win = (int)value;
+        return DS_OK;
+    }
+    if (option == DS_OPT_COUNT_FALLBACK) {
+        if (value != 0 && value != 1) return fail(c, DS_EINVAL, "DS_OPT_COUNT_FALLBACK takes 0 or 1");
+        if (c->dsc.count_fallback != (int)value) drop_graphs(c);
+        c->dsc.count_fallback = (int)value;
         return DS_OK;
     }
     if (option == DS_OPT_ADJOINT_P2G) {
@@ -1491,6 +1505,21 @@ int ds_read_stick(ds_ctx* c, int32_t inst, float* x, float* v, float* C)
     RCHK(read_species(c, inst, STICK, SX, 3, x));
     RCHK(read_species(c, inst, STICK, SV, 3, v));
     return read_species(c, inst, STICK, SC, 9, C);
+}
+
+int ds_fallback_count(ds_ctx* c, int64_t* count, int32_t reset)
+{
+    if (!c || !count) return DS_EINVAL;
+    unsigned long long n = 0;
+    HIPCHK(c, hipMemcpyAsync(&n, c->d_flag + 2, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *count = (int64_t)n;
+    if (reset) {
+        const unsigned long long z = 0;
+        HIPCHK(c, hipMemcpyAsync(c->d_flag + 2, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return DS_OK;
 }
 
 int ds_synchronize(ds_ctx* c)

@@ -35,6 +35,11 @@ struct DevScene {
     float alpha;        // -dt * vol_liq * 4 * inv_dx^2 (simulation_functions.py:150)
     float c4;           // 4 * inv_dx (simulation_functions.py:244)
     double k_bulk;
+    // tiled kernels: a stencil uses its chunk's LDS tile when its base lies within tile_win cells
+    // of the tile origin on every axis (largest 5 = tile nodes - 3); a smaller window sends more
+    // stencils to the global fall-back grid (a diagnostic; the results are the same)
+    int tile_win;
+    int count_fallback;   // 1: P2G counts its fall-back stencils (flag words 2-3, a 64-bit counter)
 };
 
 // ---------------------------------------------------------------------------- B-spline

@@ -107,10 +107,11 @@ struct Planes {
 using PlanesS = Planes<TSX, TSY>;   // scatter wave tiles, forward gathers
 using PlanesG = Planes<112, 12>;    // adjoint P2G velocity tiles (1.4 KB more LDS per plane)
 
-__device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0, int z0)
+// win: DevScene::tile_win, at most TS - 3 (the stencil's 3 nodes per axis stay inside the tile)
+__device__ __forceinline__ bool stencil_in_tile(const Stencil& s, int x0, int y0, int z0, int win)
 {
     const unsigned a = (unsigned)(s.b[0] - x0), b = (unsigned)(s.b[1] - y0), c = (unsigned)(s.b[2] - z0);
-    return a <= (unsigned)(TS - 3) && b <= (unsigned)(TS - 3) && c <= (unsigned)(TS - 3);
+    return a <= (unsigned)win && b <= (unsigned)win && c <= (unsigned)win;
 }
 
 // Compiler-only memory barrier: keeps LDS accesses in program order without waiting for
@@ -326,9 +327,10 @@ __global__ void __launch_bounds__(WG, P2GMinWaves<CH>::value) k_p2g_tiled(const 
 #pragma unroll
                     for (int a = 0; a < 9; a++) aout[a * LP + p] = pl.A[a];
                 }
-                it = stencil_in_tile(s, x0, y0, z0);
+                it = stencil_in_tile(s, x0, y0, z0, sc.tile_win);
                 if (!it) {
                     mark_fallback_blocks(fbflag, ch.inst, s, NB);
+                    if (sc.count_fallback) atomicAdd(reinterpret_cast<unsigned long long*>(flag + 2), 1ull);
                     ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
                     p2g_scatter(sc, s, pl, g);
                 }
@@ -572,7 +574,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
             Stencil s;
             if (!stencil(sc, x[0], x[1], x[2], s)) atomicOr(flag, 1u);
             float nv[3], nC[9];
-            if (stencil_in_tile(s, x0, y0, z0)) {
+            if (stencil_in_tile(s, x0, y0, z0, sc.tile_win)) {
                 TileV g{tv, x0, y0, z0};
                 g2p_gather(sc, s, g, nv, nC);
             } else {
@@ -749,7 +751,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
             float Cc[9], vc[3];
             if (valid) {
                 stencil(sc, x[0], x[1], x[2], s);
-                it = stencil_in_tile(s, x0, y0, z0);
+                it = stencil_in_tile(s, x0, y0, z0, sc.tile_win);
 #pragma unroll
                 for (int a = 0; a < 9; a++) Cc[a] = Cb[a];
 #pragma unroll
@@ -920,7 +922,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             for (int a = 0; a < 3; a++) x[a] = st[(SX + a) * LP + p];
             Stencil s;
             stencil(sc, x[0], x[1], x[2], s);
-            const bool it = stencil_in_tile(s, x0, y0, z0);
+            const bool it = stencil_in_tile(s, x0, y0, z0, sc.tile_win);
             if (ch.species == LIQUID) {
                 float xp[3];
                 if (kXp) {
@@ -1037,7 +1039,7 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_xgrad(const float* __restrict__ 
             }
 #pragma unroll
             for (int a = 0; a < 9; a++) Cb[a] = adj[(SC + a) * LP + p];
-            if (stencil_in_tile(s, x0, y0, z0)) {
+            if (stencil_in_tile(s, x0, y0, z0, sc.tile_win)) {
                 TileVT<PlanesG> gv{tv, x0, y0, z0};
                 g2p_adj_xgrad(sc, s, vt, Cb, gv, g);
             } else {

@@ -47,7 +47,7 @@ class Engine:
     # execution options of the tiled kernels (ds_set_option ids)
     OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
     OPT_OVERLAP, OPT_CHUNK_GRID, OPT_SORT_EVERY, OPT_REDUCE_PARTS, OPT_XGRAD_STREAM = 8, 9, 10, 11, 12
-    OPT_G2P_SPLIT = 13
+    OPT_G2P_SPLIT, OPT_TILE_WINDOW, OPT_COUNT_FALLBACK = 13, 14, 15
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
@@ -159,6 +159,20 @@ class Engine:
         if g2p_split is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_G2P_SPLIT, int(g2p_split)))
             self.g2p_split = int(g2p_split)
+
+    def set_diagnostics(self, tile_window: Optional[int] = None, count_fallback: Optional[bool] = None):
+        """Diagnostics of the tiled kernels: `tile_window` (0..5, default 5) narrows the window of
+        stencil bases that use their chunk's LDS tile, sending the others through the global
+        fall-back grid; `count_fallback` counts the stencils P2G sends there (fallback_count())."""
+        if tile_window is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_TILE_WINDOW, int(tile_window)))
+        if count_fallback is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_COUNT_FALLBACK, 1 if count_fallback else 0))
+
+    def fallback_count(self, reset: bool = False) -> int:
+        n = C.c_int64()
+        self._check(self.lib.ds_fallback_count(self.ctx, C.byref(n), int(bool(reset))))
+        return int(n.value)
 
     def set_kernel_timing(self, every: int):
         """Sample every N-th substep's kernels with HIP events (0: off)."""

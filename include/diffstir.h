@@ -73,6 +73,12 @@ enum {
                                           or 4 (at most the gather particles per thread): with 2 particles
                                           per thread the adjoint P2G runs 512-particle chunks while the
                                           forward's G2P runs their two 256-particle halves */
+    DS_OPT_TILE_WINDOW = 14,           /* diagnostic: a stencil uses its chunk's LDS tile when its base lies
+                                          within this many cells of the tile origin per axis, 0..5 (default
+                                          5, the largest the tile holds); others take the global fall-back
+                                          grid.  Same results, to the FP32 order of the fall-back atomics */
+    DS_OPT_COUNT_FALLBACK = 15,        /* diagnostic: 1 counts the stencils P2G sends to the fall-back grid
+                                          (ds_fallback_count); 0 (default): no counting */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or
@@ -156,6 +162,11 @@ int ds_backward(ds_ctx* ctx, const double* dl_dforces, double* dl_dparams);
 int ds_read_liquid(ds_ctx* ctx, int32_t inst, float* x, float* v, float* C, float* F);
 /* Stick state of one instance at the current step ([n][3], [n][3], [n][9]). */
 int ds_read_stick(ds_ctx* ctx, int32_t inst, float* x, float* v, float* C);
+/* Stencils P2G (forward and recompute) sent to the global fall-back grid since the last reset of the
+ * counter, when DS_OPT_COUNT_FALLBACK is on; reset != 0 clears the counter afterwards.  A diagnostic
+ * of the tiled kernels with no counterpart in the reference (which scatters every particle with
+ * global atomics, simulation_functions.py:161-163). */
+int ds_fallback_count(ds_ctx* ctx, int64_t* count, int32_t reset);
 /* Wait for all work queued on the context's stream. */
 int ds_synchronize(ds_ctx* ctx);
 

@@ -1,0 +1,217 @@
+"""The benchmark workloads on the GPU: every one against the FP64 oracle on a prefix of its
+rollout, the whole single_n64 rollout against central finite differences of its own loss, and
+the global fall-back scatter of the tiled kernels exercised where it serves.
+
+Tolerances (FP32 bulk arithmetic on the GPU against the FP64 oracle), as in test_gpu_parity:
+  forces     max_s |F - F_oracle| <= 1e-4 * max_s |F_oracle|  (rows s >= 2)
+  gradients  max_i |g_i - g_oracle_i| <= 1e-3 * max_i |g_oracle_i|
+The loss of the oracle comparisons is the workload's own (the x/z force MSE) against half the
+oracle's force profile, so dL/dforces is non-zero on every step.
+"""
+import numpy as np
+import pytest
+
+from diffstir_amd import scene as S
+from diffstir_amd.engine import Engine
+from diffstir_amd.loss import batch_loss, compute_loss
+from oracle.oracle import Oracle, loss_and_grad
+from tests.helpers import force_err, gold_scene, grad_err
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+# the execution options the autotuner picks on the benchmark workloads (a sort interval that
+# divides the checkpoint interval of the test, 4 rows per wavefront, 2 gather particles per
+# thread, split G2P workgroups, 2 reduction wavefronts, the position gradient on its own stream)
+TUNED = dict(particles_per_thread=4, gather_particles_per_thread=2, g2p_split=2, reduce_parts=2, xgrad_stream=True)
+
+
+@pytest.fixture(scope="module")
+def o64():
+    return Oracle("f64")
+
+
+_ORACLE_CACHE = {}
+
+
+def oracle_reference(o64, key, sc, nsteps, ckpt):
+    """Oracle forces, target (half of them) and dL/dparams of the workload loss; cached per key."""
+    if key not in _ORACLE_CACHE:
+        print(f"  oracle: {sc.n_liquid} liquid particles x {nsteps} substeps (threads {o64.set_threads()})",
+              flush=True)
+        F = o64.rollout(sc, nsteps=nsteps)
+        tgt = 0.5 * F
+        _, g = loss_and_grad(F, tgt, sc.cfg.loss_kind)
+        _, dth = o64.rollout_grad(sc, g, nsteps=nsteps, ckpt=ckpt)
+        _ORACLE_CACHE[key] = (F, tgt, dth)
+    return _ORACLE_CACHE[key]
+
+
+def gpu_forces_and_grad(scenes, nsteps, targets, ckpt, options=None, graph=False, sort_every=None):
+    with Engine(scenes, ckpt_every=ckpt, max_steps=nsteps + 1) as e:
+        if options:
+            e.configure(**options)
+        if sort_every:
+            e.configure(sort_every=sort_every)
+        e.set_graph(graph)
+        e.reset()
+        e.forward()
+        F = e.forces()
+        _, g = batch_loss(F, np.stack(targets), scenes[0].cfg.loss_kind)
+        d = e.backward(g)
+    return F, d
+
+
+def _lowest_fill(scenes):
+    return int(np.argmin([s.n_liquid for s in scenes]))
+
+
+CASES = {
+    # name: (workload, instance picker, substeps, checkpoint interval)
+    "single_n64": ("single_n64", lambda sc: [0], 48, 16),
+    "batch32_n64": ("batch32_n64", lambda sc: [0, _lowest_fill(sc)], 48, 16),
+    "ident16_n64": ("ident16_n64", lambda sc: [0], 48, 16),
+    "large_n128": ("large_n128", lambda sc: [0], 8, 4),
+}
+
+
+@pytest.mark.parametrize("opts", ["default", "tuned"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_workload_prefix_matches_the_oracle(o64, case, opts):
+    """Each benchmark workload's instances (the batch: its first and its lowest-fill instance,
+    advanced together in one context) on a prefix of the rollout, with the default execution
+    options and with the tuned set (hipGraph replay, a sort inside every checkpoint interval)."""
+    wl, pick, n, ckpt = CASES[case]
+    all_sc = S.workload(wl)
+    idx = pick(all_sc)
+    scenes = [all_sc[i].truncated(n + 1) for i in idx]
+    print(f"[gpu] workload {case} instances {idx} ({[s.n_liquid for s in scenes]} liquid), {n} substeps, "
+          f"checkpoint {ckpt}, options {opts}", flush=True)
+    refs = [oracle_reference(o64, (case, i), sc, n, ckpt) for i, sc in zip(idx, scenes)]
+    tuned = opts == "tuned"
+    F, d = gpu_forces_and_grad(scenes, n, [r[1] for r in refs], ckpt, options=TUNED if tuned else None,
+                               graph=tuned, sort_every=ckpt // 2 if tuned else None)
+    for b, (i, r) in enumerate(zip(idx, refs)):
+        fe = force_err(F[b], r[0])
+        ge = grad_err(d[b], r[2])
+        print(f"  instance {i}: force err {fe:.2e}  grad err {ge:.2e}  grad {d[b]}", flush=True)
+        assert fe <= FORCE_TOL
+        assert ge <= GRAD_TOL
+
+
+def test_single_n64_force_profile_over_the_whole_rollout(o64):
+    """The single_n64 force profile against the FP64 oracle over all 2000 substeps, reported every
+    200 substeps."""
+    n = 2000
+    sc = S.workload("single_n64")[0]
+    print(f"[gpu] single_n64 force profile, {n} substeps", flush=True)
+    Fo = o64.rollout(sc, nsteps=n)
+    with Engine([sc], ckpt_every=50) as e:
+        e.configure(**TUNED)
+        e.reset()
+        e.forward()
+        Fg = e.forces()[0]
+    scale = np.abs(Fo[2:]).max()
+    curve = [(s, float(np.abs(Fg[2:s + 1] - Fo[2:s + 1]).max() / scale)) for s in range(200, n + 1, 200)]
+    print("  substeps: max relative force error so far " + ", ".join(f"{s}: {e:.2e}" for s, e in curve), flush=True)
+    assert curve[-1][1] <= FORCE_TOL
+
+
+def test_single_n64_full_rollout_gradient_matches_central_differences():
+    """The whole single_n64 rollout (2000 substeps, checkpoint every 50, tuned options, hipGraph
+    replay): the adjoint's dL/dparams against central differences of the GPU's own loss with a
+    relative step of 1e-3 per parameter (the loss against the force profile at the ground-truth
+    parameters, as the benchmark computes it)."""
+    sc = S.workload("single_n64")[0]
+    truth = S.hb_params_draw(np.random.default_rng(S.GROUND_TRUTH_SEED), sc.params[3])
+    theta = S.hb_params_draw(np.random.default_rng(7), sc.params[3])
+    h = 1e-3
+    print(f"[gpu] single_n64 full-rollout central differences, theta {theta}", flush=True)
+    kind = sc.cfg.loss_kind
+    with Engine([sc], ckpt_every=50) as e:
+        e.configure(**TUNED, sort_every=25)
+        e.set_graph(True)
+
+        def loss_at(p):
+            e.set_params(np.asarray(p)[None])
+            e.reset()
+            e.forward()
+            return e.forces()[0]
+
+        tgt = loss_at(truth)
+        F = loss_at(theta)
+        L, g = compute_loss(F, tgt, kind)
+        d = e.backward(g[None])[0]
+        fd = np.zeros(4)
+        for j in range(4):
+            tp, tm = theta.copy(), theta.copy()
+            tp[j] *= 1 + h
+            tm[j] *= 1 - h
+            Lp, _ = compute_loss(loss_at(tp), tgt, kind)
+            Lm, _ = compute_loss(loss_at(tm), tgt, kind)
+            fd[j] = (Lp - Lm) / (2 * h * theta[j])
+    rel = np.abs(d - fd) / np.maximum(np.abs(fd), 1e-300)
+    print(f"  loss {L:.6g}\n  adjoint {d}\n  central {fd}\n  relative error per parameter {rel}", flush=True)
+    # measured on MI355X: 5.2e-4, 7.7e-4, 3.1e-5, 1.6e-4 (central differences carry O(h^2)
+    # truncation and the FP32 forward's rounding)
+    assert np.all(rel <= 3e-3)
+    assert grad_err(d, fd) <= 1e-3
+
+
+# ----------------------------------------------------------------------------- fall-back scatter
+
+@pytest.mark.parametrize("window", [0, 1, 3])
+def test_fallback_scatter_matches_the_fixture(window):
+    """The tiled kernels with a narrowed tile window: most stencils go through the global
+    fall-back grid (P2G, the adjoint G2P, and the gathers read the global grids).  The full
+    18.5k-particle n_grid-32 scene over 20 substeps against its committed oracle fixture."""
+    sc, z = gold_scene("plain_n32_20")
+    n = int(z["nsteps"])
+    print(f"[gpu] fall-back scatter, tile window {window}", flush=True)
+    with Engine([sc], ckpt_every=7, max_steps=n + 1) as e:
+        e.set_diagnostics(tile_window=window, count_fallback=True)
+        e.reset()
+        e.forward()
+        F = e.forces()[0]
+        L, g = compute_loss(F, z["target"], sc.cfg.loss_kind)
+        d = e.backward(g[None])[0]
+        nfb = e.fallback_count()
+    # P2G runs 2n times (forward and recompute) over all particles
+    share = nfb / (2 * n * (sc.n_liquid + sc.n_stick + sc.n_container))
+    fe, ge = force_err(F, z["forces"]), grad_err(d, z["dl_dparams"])
+    print(f"  fall-back stencils {nfb} ({share:.1%} of P2G stencils), force err {fe:.2e}, grad err {ge:.2e}",
+          flush=True)
+    assert share > (0.5 if window <= 1 else 0.05)
+    assert fe <= FORCE_TOL
+    assert ge <= GRAD_TOL
+
+
+def test_large_n128_drift_takes_the_fallback_path():
+    """large_n128 with its stated checkpoint interval of 100 and one sort per interval: particles
+    drift out of their chunk's tile within the segment and take the fall-back scatter.  The tiled
+    kernels against the per-particle kernels (one thread per particle, global atomics) on the
+    first 200 substeps."""
+    n, ckpt = 200, 100
+    sc = S.workload("large_n128")[0].truncated(n + 1)
+    print(f"[gpu] large_n128 {sc.n_liquid} liquid, {n} substeps, tiled against per-particle", flush=True)
+    out = {}
+    for kern in ("per_particle", "tiled"):
+        with Engine([sc], ckpt_every=ckpt, kernels=kern) as e:
+            if kern == "tiled":
+                e.set_diagnostics(count_fallback=True)
+            e.reset()
+            e.forward()
+            F = e.forces()[0]
+            tgt = out["per_particle"][0] * 0.5 if kern == "tiled" else 0.5 * F
+            _, g = compute_loss(F, tgt, sc.cfg.loss_kind)
+            d = e.backward(g[None])[0]
+            out[kern] = (F, d, e.fallback_count() if kern == "tiled" else 0)
+    Fp, dp, _ = out["per_particle"]
+    Ft, dt_, nfb = out["tiled"]
+    fe, ge = force_err(Ft, Fp), grad_err(dt_, dp)
+    print(f"  fall-back stencils {nfb}, force err {fe:.2e}, grad err {ge:.2e}", flush=True)
+    assert nfb > 0
+    assert fe <= FORCE_TOL
+    assert ge <= GRAD_TOL

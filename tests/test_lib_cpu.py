@@ -116,7 +116,8 @@ def test_engine_option_numbers_match_the_header():
             "OPT_ADJ_P2G": "DS_OPT_ADJOINT_P2G", "OPT_SCATTER": "DS_OPT_SCATTER_CHUNKS", "OPT_GRAPH": "DS_OPT_GRAPH",
             "OPT_OVERLAP": "DS_OPT_OVERLAP", "OPT_CHUNK_GRID": "DS_OPT_CHUNK_GRID",
             "OPT_SORT_EVERY": "DS_OPT_SORT_EVERY", "OPT_REDUCE_PARTS": "DS_OPT_REDUCE_PARTS",
-            "OPT_XGRAD_STREAM": "DS_OPT_XGRAD_STREAM", "OPT_G2P_SPLIT": "DS_OPT_G2P_SPLIT"}
+            "OPT_XGRAD_STREAM": "DS_OPT_XGRAD_STREAM", "OPT_G2P_SPLIT": "DS_OPT_G2P_SPLIT",
+            "OPT_TILE_WINDOW": "DS_OPT_TILE_WINDOW", "OPT_COUNT_FALLBACK": "DS_OPT_COUNT_FALLBACK"}
     for attr, name in want.items():
         assert getattr(Engine, attr) == enum[name], name
     assert len([k for k in enum if k.startswith("DS_OPT_")]) == len(want)
