@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libdiffstir_hip.so")
 EXPORTS = (
     "ds_create", "ds_destroy", "ds_set_option", "ds_last_error", "ds_set_particles", "ds_set_trajectory", "ds_set_params",
     "ds_reset", "ds_forward", "ds_current_step", "ds_get_forces", "ds_backward", "ds_read_liquid",
-    "ds_read_stick", "ds_fallback_count", "ds_synchronize", "ds_last_timing", "ds_kernel_timing", "ds_svd3", "ds_svd3_values", "ds_svd3_backward",
+    "ds_read_stick", "ds_fallback_count", "ds_substep_noalloc", "ds_synchronize", "ds_last_timing", "ds_kernel_timing", "ds_svd3", "ds_svd3_values", "ds_svd3_backward",
 )
 
 KERNEL_NAMES = ("k_p2g_tiled", "k_grid_reduce", "k_g2p_tiled", "k_g2p_adj_tiled", "k_vbar_reduce",
@@ -78,6 +78,8 @@ def load() -> C.CDLL:
         "ds_read_liquid": (C.c_int, [vp, C.c_int32, F32P, F32P, F32P, F32P]),
         "ds_read_stick": (C.c_int, [vp, C.c_int32, F32P, F32P, F32P]),
         "ds_fallback_count": (C.c_int, [vp, C.POINTER(C.c_int64), C.c_int32]),
+        "ds_substep_noalloc": (C.c_int, [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, vp,
+                                         vp, vp]),
         "ds_synchronize": (C.c_int, [vp]),
         "ds_last_timing": (C.c_int, [vp, F64P, F64P]),
         "ds_kernel_timing": (C.c_int, [vp, F64P, C.POINTER(C.c_int64), C.c_int32]),

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1098,6 +1099,76 @@ int read_species(ds_ctx* c, int inst, int sp, int slot0, int ncomp, float* out)
     return DS_OK;
 }
 
+// ---------------------------------------------------------------------------- no-allocation substep
+
+// Scratch of ds_substep_noalloc: one instance's dense grid (momentum/mass, then velocity), the
+// slot-to-group table, the group table, the stick's dv sums and the failure flag.  One copy per
+// GPU, held until the process ends; every call on a GPU shares it.
+constexpr int NA_MAX_GRID = 64;                                   // nodes per axis
+constexpr long NA_MAX_NODES = (long)NA_MAX_GRID * NA_MAX_GRID * NA_MAX_GRID;
+constexpr long NA_MAX_SLOTS = 1l << 18;                           // particle slots (runs padded to 64)
+constexpr size_t NA_GRID_BYTES = sizeof(float4) * NA_MAX_NODES;  // 4 MiB each
+constexpr size_t NA_META_BYTES = sizeof(int) * NA_MAX_SLOTS;     // 1 MiB
+constexpr size_t NA_TAB_BYTES = 256;                              // groups 48 B, dv sums at 64, flag at 128
+constexpr size_t NA_SCRATCH_BYTES = 2 * NA_GRID_BYTES + NA_META_BYTES + NA_TAB_BYTES;
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[NA_MAX_DEVICES] = {};   // device address of `scratch` per device
+
+// The device address of the scratch array on the current device, looked up once per device.
+int na_scratch(void** out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, DS_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(nullptr, DS_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(nullptr, DS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_na_ptr[dev] = ptr;
+    }
+    *out = g_na_ptr[dev];
+    return DS_OK;
+}
+
+// Slot-to-group table of one instance (liquid, stick, container runs, each padded to 64), the
+// group table, cleared dv sums and flag.
+__global__ void __launch_bounds__(256) k_na_setup(int* __restrict__ meta, Group* __restrict__ groups,
+                                                  double* __restrict__ dvsum, unsigned* __restrict__ flag, int nl,
+                                                  int ns, int nc, int P)
+{
+    const int o1 = (nl + 63) / 64 * 64, o2 = o1 + (ns + 63) / 64 * 64;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
+        int g = -1;
+        if (p < nl) g = 0;
+        else if (p >= o1 && p < o1 + ns) g = 1;
+        else if (p >= o2 && p < o2 + nc) g = 2;
+        meta[p] = g;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        groups[0] = Group{0, LIQUID, 0, nl};
+        groups[1] = Group{0, STICK, o1, ns};
+        groups[2] = Group{0, CONTAINER, o2, nc};
+        for (int i = 0; i < 6; i++) dvsum[i] = 0.0;
+        flag[0] = 0u;
+    }
+}
+
+// Force of the substep (m_stick / dt * sum dv, the row ds_get_forces reports) and the flag.
+__global__ void k_na_finish(const double* __restrict__ dvsum, const unsigned* __restrict__ flag, double scale,
+                            double* __restrict__ force_out, uint32_t* __restrict__ status_out)
+{
+    if (threadIdx.x < 3) force_out[threadIdx.x] = scale * dvsum[3 + threadIdx.x];
+    if (threadIdx.x == 0) status_out[0] = flag[0] ? 1u : 0u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1505,6 +1576,52 @@ int ds_read_stick(ds_ctx* c, int32_t inst, float* x, float* v, float* C)
     RCHK(read_species(c, inst, STICK, SX, 3, x));
     RCHK(read_species(c, inst, STICK, SV, 3, v));
     return read_species(c, inst, STICK, SC, 9, C);
+}
+
+int ds_substep_noalloc(const ds_scene_desc* desc, int32_t n_liquid, int32_t n_stick, int32_t n_container,
+                       const double* params, const float* state_in, float* state_out, const float* traj2,
+                       double* force_out, uint32_t* status_out, void* stream)
+{
+    if (!desc || !params || !state_in || !state_out || !traj2 || !force_out || !status_out)
+        return fail(nullptr, DS_EINVAL, "NULL argument");
+    if (desc->n_grid < 8 || desc->n_grid > NA_MAX_GRID)
+        return fail(nullptr, DS_EINVAL, "n_grid out of the no-allocation range [8, 64]");
+    if (desc->model < 0 || desc->model > 2) return fail(nullptr, DS_EINVAL, "model must be 0, 1 or 2");
+    if (desc->k_from_param && desc->model != DS_MODEL_HERSCHEL_BULKLEY)
+        return fail(nullptr, DS_EINVAL, "k_from_param needs the Herschel-Bulkley model (params[3] is free only there)");
+    if (!(desc->dt > 0.0)) return fail(nullptr, DS_EINVAL, "dt must be > 0");
+    if (n_liquid < 0 || n_stick < 0 || n_container < 0) return fail(nullptr, DS_EINVAL, "negative particle count");
+    const long P = ((long)n_liquid + 63) / 64 * 64 + ((long)n_stick + 63) / 64 * 64 + ((long)n_container + 63) / 64 * 64;
+    if (P > NA_MAX_SLOTS)
+        return fail(nullptr, DS_EINVAL, "more particle slots than the no-allocation limit 262144 (runs padded to 64)");
+    if (P == 0) return fail(nullptr, DS_EINVAL, "no particles");
+    void* base = nullptr;
+    RCHK(na_scratch(&base));
+    unsigned char* b = static_cast<unsigned char*>(base);
+    float4* grid = reinterpret_cast<float4*>(b);
+    float4* vout = reinterpret_cast<float4*>(b + NA_GRID_BYTES);
+    int* meta = reinterpret_cast<int*>(b + 2 * NA_GRID_BYTES);
+    unsigned char* tab = b + 2 * NA_GRID_BYTES + NA_META_BYTES;
+    Group* groups = reinterpret_cast<Group*>(tab);
+    double* dvsum = reinterpret_cast<double*>(tab + 64);
+    unsigned* flag = reinterpret_cast<unsigned*>(tab + 128);
+    const DevScene dsc = make_dev_scene(*desc);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const long nodes = (long)desc->n_grid * desc->n_grid * desc->n_grid;
+    const int Pi = (int)P;
+    hipLaunchKernelGGL(k_na_setup, dim3(nblocks(P)), dim3(256), 0, st, meta, groups, dvsum, flag, (int)n_liquid,
+                       (int)n_stick, (int)n_container, Pi);
+    hipLaunchKernelGGL(k_fill_f4, dim3(nblocks(nodes)), dim3(256), 0, st, grid, nodes);
+    hipLaunchKernelGGL(k_p2g_ref, dim3(nblocks(P)), dim3(256), 0, st, state_in, state_out, meta,
+                       (const Group*)groups, params, grid, dsc, Pi, flag);
+    hipLaunchKernelGGL(k_gridop_ref, dim3(nblocks(nodes)), dim3(256), 0, st, (const float4*)grid, vout, dsc, 1);
+    hipLaunchKernelGGL(k_g2p_ref, dim3(nblocks(P)), dim3(256), 0, st, state_in, state_out, meta,
+                       (const Group*)groups, (const float4*)vout, traj2, 2, 0, dvsum, dsc, Pi, flag);
+    hipLaunchKernelGGL(k_na_finish, dim3(1), dim3(64), 0, st, (const double*)dvsum, (const unsigned*)flag,
+                       desc->m_stick / desc->dt, force_out, status_out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(nullptr, DS_EHIP, std::string("ds_substep_noalloc launch: ") + hipGetErrorString(e));
+    return DS_OK;
 }
 
 int ds_fallback_count(ds_ctx* c, int64_t* count, int32_t reset)

@@ -167,6 +167,28 @@ int ds_read_stick(ds_ctx* ctx, int32_t inst, float* x, float* v, float* C);
  * of the tiled kernels with no counterpart in the reference (which scatters every particle with
  * global atomics, simulation_functions.py:161-163). */
 int ds_fallback_count(ds_ctx* ctx, int64_t* count, int32_t reset);
+/* One substep of one instance on caller-owned device memory, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is the per-particle form of
+ * ds_forward (DS_KERNELS_PER_PARTICLE, one instance, one substep): P2G with global atomics into a
+ * dense grid, the grid update, G2P.  Every pointer but desc is device memory:
+ *   params      [4] doubles (constitutive_parameters)
+ *   state_in    [24][P] floats, component-major: x 0-2, v 3-5, C 6-14 (row-major), F 15-23 for P
+ *               slots = the liquid, stick and container runs in that order, each padded to a
+ *               multiple of 64 (padding slots are ignored); stick and container F are not read
+ *   state_out   [24][P], the state after the substep (may be state_in)
+ *   traj2       [2][3] floats: the stick trajectory frames f and f+1
+ *   force_out   [3] doubles: the substep's stick force m_stick/dt * sum_p dv (the row f+1 that
+ *               ds_get_forces reports)
+ *   status_out  [1]: 0, or 1 if a particle left the grid's interior or became non-finite
+ * Limits: n_grid <= 64 and P <= 262144 slots (DS_EINVAL beyond them).  Its grid, tables and flag
+ * live in one 9 MiB device array per GPU, held until the process ends; the call clears what must
+ * start at zero.  Calls that overlap on one GPU race on that array.  The work is queued on
+ * `stream` (a hipStream_t; NULL is the null stream) and the call returns without waiting.
+ * Argument errors return DS_EINVAL with the message in ds_last_error(NULL). */
+int ds_substep_noalloc(const ds_scene_desc* desc, int32_t n_liquid, int32_t n_stick, int32_t n_container,
+                       const double* params, const float* state_in, float* state_out, const float* traj2,
+                       double* force_out, uint32_t* status_out, void* stream);
 /* Wait for all work queued on the context's stream. */
 int ds_synchronize(ds_ctx* ctx);
 

@@ -125,3 +125,19 @@ def test_engine_option_numbers_match_the_header():
     assert Engine.ADJ_P2G == {"fused": enum["DS_ADJ_P2G_FUSED"], "split": enum["DS_ADJ_P2G_SPLIT"]}
     assert Engine.SCATTER == {"rows": enum["DS_SCATTER_ROWS"], "packed": enum["DS_SCATTER_PACKED"]}
     assert len(_lib.KERNEL_NAMES) == enum["DS_NKERNELS"]
+
+
+def test_noalloc_substep_rejects_bad_arguments_on_the_host():
+    """ds_substep_noalloc validates its sizes before touching the device: beyond its scratch
+    (n_grid above 64, more than 262144 padded slots) or with NULL buffers it returns DS_EINVAL."""
+    lib = _lib.load()
+    desc = scene_desc(S.tiny_scene(16, 2).cfg)
+    p = C.c_void_p(4096)   # never dereferenced: every case fails validation first
+    desc.n_grid = 128
+    assert lib.ds_substep_noalloc(C.byref(desc), 10, 10, 0, p, p, p, p, p, p, None) == 1
+    assert b"no-allocation" in lib.ds_last_error(None)
+    desc.n_grid = 16
+    assert lib.ds_substep_noalloc(C.byref(desc), 262144, 1, 0, p, p, p, p, p, p, None) == 1
+    assert b"no-allocation limit" in lib.ds_last_error(None)
+    assert lib.ds_substep_noalloc(C.byref(desc), 10, 10, 0, p, None, p, p, p, p, None) == 1
+    assert lib.ds_substep_noalloc(C.byref(desc), -1, 10, 0, p, p, p, p, p, p, None) == 1
