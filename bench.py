@@ -12,7 +12,9 @@ Prints one JSON line:
   value      fwd+adjoint particle-steps/s (liquid particles x substeps x evaluations / s)
   extra      gradient evals/s, forward-only and adjoint split, the batched workload figure
   roofline   the dominant kernel's algorithmic HBM bytes / its average launch time
-  cpu_baseline  the FP64 CPU oracle (single thread) on a bounded sample of the same workload
+  cpu_baseline  the reference's structure on the host (FP32, per-particle parallel loops with atomic
+             grid adds, OpenMP on the host's cores; and one core) on a bounded sample of the same
+             workload; the FP64 oracle checks the GPU on that sample
 """
 from __future__ import annotations
 
@@ -33,6 +35,15 @@ from diffstir_amd.engine import Engine  # noqa: E402
 from diffstir_amd.loss import batch_loss  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# A HIP event pair on a launch (hipExtLaunchKernel) reads the kernel's execution plus the gap with
+# which the dispatch follows the packet that records the start event (untimed launches follow each
+# other with no gap): 4.0 us on MI355X, the median over the seven tiled kernels of (event sample -
+# traced duration of the same dispatches) in profiles/event_timing_single_n64.md.
+DISPATCH_GAP_US = 4.0
+# the autotuner's choices per workload signature (diffstir_amd.autotune.signature), written by
+# `bench.py --retune` on an MI355X and read by default runs, so that runs and profiles of the
+# same command use the same kernels
+TUNED_FILE = os.path.join(ROOT, "diffstir_amd", "tuned_options.json")
 FWD_BYTES = 192.0            # FP32 x, v, C, F read once and written once per particle-step
 
 # Algorithmic HBM bytes per particle (all species) of one launch of each tiled kernel: the
@@ -93,8 +104,16 @@ def device_free_bytes() -> int:
 
 def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "tiled", timing_every: int = 10,
             autotune: bool = True, graph: bool = True, kernel_sample: bool = True, substeps: int = 0,
-            overlap: bool = True, sort_every: Optional[int] = None, reduce_parts: Optional[int] = None):
+            overlap: bool = True, sort_every: Optional[int] = None, reduce_parts: Optional[int] = None,
+            options: Optional[dict] = None, retune: bool = False, warm_start: int = 0):
     scenes = S.workload(workload)
+    if warm_start:
+        # counter runs: start from the positions the rollout reaches after `warm_start` substeps
+        # (the stirred state the autotuner also times), velocities from rest
+        from diffstir_amd.autotune import warmed_scenes
+        with Engine(scenes, ckpt_every=ckpt) as tmp:
+            scenes = warmed_scenes(scenes, tmp, warm_start)
+        log(f"[bench] warm start: positions after {warm_start} substeps")
     if substeps:
         # a shortened rollout (profiling runs whose every launch is serialised by the profiler)
         scenes = [s.truncated(min(substeps + 1, s.max_steps)) for s in scenes]
@@ -106,11 +125,28 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
         f"checkpoint every {ckpt}")
     eng = Engine(scenes, ckpt_every=ckpt, kernels=kernels, overlap=overlap)
     tuned = None
-    if autotune and kernels == "tiled":
+    if options and kernels == "tiled":
+        # fixed execution options (profiling runs: no tuner candidates among the traced launches)
+        eng.configure(**options)
+        log(f"[bench] fixed options: {options}")
+    elif autotune and kernels == "tiled":
         from diffstir_amd.autotune import tune
         t0 = time.perf_counter()
-        tuned = tune(scenes, ckpt_every=ckpt, engine=eng)
-        log(f"[bench] autotuned in {time.perf_counter() - t0:.1f} s: {tuned['options']}")
+        if retune:
+            # a fresh choice, stored in the options file the default runs read
+            tuned = tune(scenes, ckpt_every=ckpt, engine=eng)
+            disk = {}
+            if os.path.exists(TUNED_FILE):
+                with open(TUNED_FILE) as f:
+                    disk = json.load(f)
+            disk[tuned["signature"]] = tuned
+            with open(TUNED_FILE, "w") as f:
+                json.dump(disk, f, indent=1)
+        else:
+            # the committed choice for this workload when there is one, else a fresh one
+            tuned = tune(scenes, ckpt_every=ckpt, engine=eng, cache_file=TUNED_FILE)
+            eng.configure(**tuned["options"])
+        log(f"[bench] autotuned options ({time.perf_counter() - t0:.1f} s): {tuned['options']}")
     if sort_every is not None:
         eng.configure(sort_every=sort_every)
     if reduce_parts is not None:
@@ -151,7 +187,8 @@ def run_gpu(workload: str, steps: int, warmup: int, ckpt: int, kernels: str = "t
     psteps = float(n_liq) * (T - 1) * steps
     res = dict(workload=workload, scenes=scenes, n_liq=n_liq, n_all=n_all, T=T, elapsed=elapsed, psteps=psteps,
                fwd_ms=fwd_ms / steps, bwd_ms=bwd_ms / steps, B=len(scenes), kernels=kt,
-               options=eng.options() if kernels == "tiled" else None, tuned=tuned, mem_used=mem_used, graph=graph)
+               options=eng.options() if kernels == "tiled" else None, tuned=tuned, mem_used=mem_used, graph=graph,
+               timing_every=timing_every)
     eng.close()
     return res
 
@@ -178,25 +215,47 @@ def plain_single(workload: str, sample_steps: int, ckpt: int) -> dict:
             "fwd_adjoint_particle_steps_per_s": sc.n_liquid * sample_steps / el}
 
 
-def cpu_baseline(workload: str, sample_steps: int, ckpt: int, check_workloads=()):
-    """The FP64 oracle, one thread, on the first `sample_steps` substeps of the workload's first
-    instance (forward + checkpointed adjoint).  The oracle's forces and parameter gradient on
-    that sample also check the GPU: the tiled kernels run the same sample and their relative
-    errors are reported, for `workload` and for instance 0 of each (workload, substeps) of
-    `check_workloads`."""
+def cpu_baseline(workload: str, sample_steps: int, ckpt: int, one_core_steps: int = 4, check_workloads=()):
+    """The reference's execution structure on the host: the FP32 restatement of the substep with
+    one parallel loop over the particles of each species and atomic adds into a dense grid (as
+    Taichi runs simulation_functions.py:104-290 on ti.cpu; oracle/mpm_oracle.c built with
+    ORACLE_ATOMIC), OpenMP on the host's cores, on the first `sample_steps` substeps of the
+    workload's first instance: the checkpointed forward + adjoint (the forward with checkpoints,
+    each segment's recompute and the step adjoints), timed alone.  The same on one core over
+    `one_core_steps` substeps.  The FP64 oracle then checks the GPU on the same sample (not timed):
+    the tiled kernels run it and their relative errors are reported, for `workload` and for
+    instance 0 of each (workload, substeps) of `check_workloads`."""
     from oracle.oracle import Oracle, loss_and_grad
-    o = Oracle("f64")
+    ref = Oracle("f32at")
     sc = S.workload(workload)[0].truncated(sample_steps + 1)
+    threads = ref.set_threads(0)
+    # dL/dforces of the sample's loss (target: half the forces), from an untimed forward
+    F = ref.rollout(sc, nsteps=sample_steps)
+    _, g = loss_and_grad(F, 0.5 * F, sc.cfg.loss_kind)
     t0 = time.perf_counter()
-    F = o.rollout(sc, nsteps=sample_steps)
-    L, g = loss_and_grad(F, 0.5 * F, sc.cfg.loss_kind)
-    _, dth = o.rollout_grad(sc, g, nsteps=sample_steps, ckpt=min(ckpt, sample_steps))
+    ref.rollout_grad(sc, g, nsteps=sample_steps, ckpt=min(ckpt, sample_steps))
     el = time.perf_counter() - t0
-    # the timed region ran the forward twice (plain + inside rollout_grad); count one fwd+adj
-    out = dict(value=sc.n_liquid * sample_steps / el, unit="fwd+adjoint particle-steps/s", cores=1, kind="port",
-               sample=f"{workload} instance 0, first {sample_steps} substeps, forward + checkpointed adjoint "
-                      f"(plus one extra forward), FP64 oracle, {el:.1f} s")
-    checks = {workload: gpu_vs_oracle(sc, F, dth, ckpt)}
+    sc1 = S.workload(workload)[0].truncated(one_core_steps + 1)
+    F1 = ref.rollout(sc1, nsteps=one_core_steps)
+    _, g1 = loss_and_grad(F1, 0.5 * F1, sc1.cfg.loss_kind)
+    ref.set_threads(1)
+    t0 = time.perf_counter()
+    ref.rollout_grad(sc1, g1, nsteps=one_core_steps, ckpt=min(ckpt, one_core_steps))
+    el1 = time.perf_counter() - t0
+    ref.set_threads(threads)
+    out = dict(value=sc.n_liquid * sample_steps / el, unit="fwd+adjoint particle-steps/s", cores=threads,
+               kind="port",
+               sample=f"{workload} instance 0 ({sc.n_liquid} liquid particles), first {sample_steps} substeps, "
+                      f"checkpointed forward + adjoint, FP32 with the reference's structure (per-particle "
+                      f"parallel loops, atomic grid adds), {threads} OpenMP threads, {el:.1f} s",
+               host_cpus=os.cpu_count(),
+               one_core={"value": sc1.n_liquid * one_core_steps / el1, "substeps": one_core_steps,
+                         "seconds": el1})
+    o = Oracle("f64")
+    Fo = o.rollout(sc, nsteps=sample_steps)
+    _, go = loss_and_grad(Fo, 0.5 * Fo, sc.cfg.loss_kind)
+    _, dth = o.rollout_grad(sc, go, nsteps=sample_steps, ckpt=min(ckpt, sample_steps))
+    checks = {workload: gpu_vs_oracle(sc, Fo, dth, ckpt)}
     for wl, n in check_workloads:
         sb = S.workload(wl)[0].truncated(n + 1)
         Fb = o.rollout(sb, nsteps=n)
@@ -226,7 +285,13 @@ def gpu_vs_oracle(sc, F_o, dth_o, ckpt) -> dict:
 
 def roofline(r) -> dict:
     """The dominant tiled kernel (largest sampled GPU time): its algorithmic bytes per launch
-    divided by its average launch time, both from this run."""
+    divided by its average launch time, both from this run.  The samples are HIP event pairs on
+    the kernel's own launch (hipExtLaunchKernel on its stream), every `timing_every`-th substep of
+    one launched evaluation after the timed region, each sampled substep starting behind a short
+    hold of the stream so that the GPU never waits for the host.  A sample is the kernel's
+    execution plus the dispatch gap that the event-recording packet causes, DISPATCH_GAP_US
+    (measured on MI355X against the kernel trace: profiles/event_timing_single_n64.md); the
+    kernel's time is the sample minus that gap."""
     bpp = dict(KERNEL_BYTES_PER_PARTICLE)
     if r.get("options"):
         bpp["k_p2g_adj_tiled"] = ADJ_P2G_BYTES[r["options"]["adjoint_p2g"]]
@@ -235,7 +300,8 @@ def roofline(r) -> dict:
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None}
     name = max(kt, key=lambda k: kt[k][0])
     tot_ms, n = kt[name]
-    avg_s = tot_ms / n / 1e3
+    raw_us = tot_ms / n * 1e3
+    avg_s = (raw_us - DISPATCH_GAP_US) / 1e6
     per_launch = bpp[name] * r["n_all"]
     achieved = per_launch / avg_s / 1e9
     traffic = None
@@ -243,13 +309,43 @@ def roofline(r) -> dict:
     if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
-        if t.get("kernel") == name:
+        if t.get("kernel") == name and t.get("options") == r.get("options"):
             traffic = t["bytes_per_launch"]
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": name,
-            "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
-            "sampled_launches": n,
-            "kernel_avg_us": {k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in r["kernels"].items()}}
+            "avg_launch_us": avg_s * 1e6, "event_sample_us": raw_us, "dispatch_gap_us": DISPATCH_GAP_US,
+            "algorithmic_bytes_per_launch": per_launch, "sampled_launches": n,
+            "kernel_avg_us": {k: (v[0] / v[1] * 1e3 - DISPATCH_GAP_US if v[1] else None)
+                              for k, v in r["kernels"].items()}}
+
+
+def trace_summary(path: str):
+    """Per-kernel average durations from a rocprofv3 kernel-trace CSV of this script: over all
+    dispatches, and over the dispatches the kernel sampling timed with HIP events (the first
+    dispatch of each kernel after each k_hold on the same queue), in microseconds."""
+    import csv
+    import re
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    short = lambda n: re.sub(r"^void |\(.*$", "", n).replace("ds::", "")
+    allk, sampled = {}, {}
+    seen = {}
+    for r in rows:
+        name, q = short(r["Kernel_Name"]), r["Queue_Id"]
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        if name == "k_hold":
+            seen[q] = set()
+            continue
+        allk.setdefault(name, []).append(d)
+        if q in seen and name not in seen[q]:
+            seen[q].add(name)
+            sampled.setdefault(name, []).append(d)
+    out = {}
+    for name, v in sorted(allk.items(), key=lambda kv: -sum(kv[1])):
+        sv = sampled.get(name, [])
+        out[name] = {"dispatches": len(v), "avg_us": sum(v) / len(v), "total_ms": sum(v) / 1e3,
+                     "sampled": len(sv), "sampled_avg_us": (sum(sv) / len(sv)) if sv else None}
+    print(json.dumps(out, indent=1))
 
 
 def calibrate_copy(mib: int):
@@ -276,24 +372,37 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="single_n64")
     ap.add_argument("--ckpt", type=int, default=50)
-    ap.add_argument("--cpu-sample-steps", type=int, default=48)
+    ap.add_argument("--cpu-sample-steps", type=int, default=100)
+    ap.add_argument("--cpu-one-core-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels", default="tiled", choices=["tiled", "per_particle"])
     ap.add_argument("--batch-workload", default="batch32_n64")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
+    ap.add_argument("--trace-summary", default=None,
+                    help="summarise a rocprofv3 kernel-trace CSV of this script (no GPU work) and exit")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run the adjoint's segment recompute on the main stream (no second stream)")
     ap.add_argument("--substeps", type=int, default=0,
                     help="shorten the rollout to this many substeps (profiling only; 0 = the workload's own)")
+    ap.add_argument("--warm-start", type=int, default=0,
+                    help="start from the positions the rollout reaches after this many substeps (counter runs "
+                         "of a shortened rollout in the stirred state; 0 = the initial fill)")
     ap.add_argument("--sort-every", type=int, default=None,
                     help="sort the particles every this many substeps (a divisor of --ckpt; 0 = once per "
                          "checkpoint interval); overrides the autotuner's choice")
     ap.add_argument("--reduce-parts", type=int, default=None, choices=[1, 2, 4],
                     help="wavefronts per node block of the grid reductions; overrides the autotuner's choice")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the autotuner even where diffstir_amd/tuned_options.json holds a choice, and store it")
+    ap.add_argument("--options", default=None,
+                    help="JSON of fixed execution options (Engine.configure keywords) instead of the autotuner")
     a = ap.parse_args()
+    if a.trace_summary:
+        trace_summary(a.trace_summary)
+        return
     if a.calibrate_copy_mib:
         calibrate_copy(a.calibrate_copy_mib)
         return
@@ -301,8 +410,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the instances of a batch share one device "
               "and nothing here uses a second GPU", file=sys.stderr)
         sys.exit(2)
+    fixed = json.loads(a.options) if a.options else None
     r = run_gpu(a.workload, a.steps, a.warmup, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph,
-                substeps=a.substeps, overlap=not a.no_overlap, sort_every=a.sort_every, reduce_parts=a.reduce_parts)
+                substeps=a.substeps, overlap=not a.no_overlap, sort_every=a.sort_every, reduce_parts=a.reduce_parts,
+                options=fixed, retune=a.retune, warm_start=a.warm_start)
     value = r["psteps"] / r["elapsed"]
     ms_per_step = 1000.0 * r["elapsed"] / a.steps
     bpps = fwd_adj_bytes(a.ckpt)
@@ -322,6 +433,7 @@ def main():
         "data": "synthetic (seeded cylinder fill, circular stick path)",
         "config": {"workload": a.workload, "instances": r["B"], "liquid_particles": r["n_liq"],
                    "substeps": r["T"] - 1, "n_grid": r["scenes"][0].cfg.n_grid, "ckpt_every": a.ckpt,
+                   "density": r["scenes"][0].cfg.m_liq / r["scenes"][0].cfg.vol_liq,
                    "kernels": a.kernels, "options": r["options"], "autotuned": r["tuned"] is not None,
                    "hip_graph": r["graph"]},
         "extra": {"gradient_evals_per_s": r["B"] * a.steps / r["elapsed"], "fwd_ms": r["fwd_ms"],
@@ -332,7 +444,7 @@ def main():
     }
     if not a.no_batch:
         rb = run_gpu(a.batch_workload, 1, 1, a.ckpt, a.kernels, autotune=not a.no_autotune, graph=not a.no_graph,
-                     kernel_sample=False, overlap=not a.no_overlap)
+                     kernel_sample=False, overlap=not a.no_overlap, retune=a.retune)
         line["extra"]["batched"] = {
             "workload": a.batch_workload, "instances": rb["B"], "liquid_particles": rb["n_liq"],
             "options": rb["options"],
@@ -349,7 +461,8 @@ def main():
         # identification scene (Herschel-Bulkley, 3000 substeps) and the n_grid 128 instance
         checks = () if a.no_batch else ((a.batch_workload, a.cpu_sample_steps), ("ident16_n64", a.cpu_sample_steps),
                                         ("large_n128", 4))
-        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_sample_steps, a.ckpt, check_workloads=checks)
+        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_sample_steps, a.ckpt, a.cpu_one_core_steps,
+                                            check_workloads=checks)
     print(json.dumps(line), flush=True)
 
 

@@ -298,7 +298,10 @@ hipEvent_t kt_event(ds_ctx* c)
 // default): events recorded on the stream before and after the scope's launches, so the time
 // includes the dispatch gaps between them.  Dispatch form (`dispatch` true): the scope makes one
 // launch through DS_LAUNCH_T / DS_PPT_DISPATCH*_T, which hands both events to
-// hipExtLaunchKernel, and the time is the kernel's own execution.
+// hipExtLaunchKernel.  Measured on MI355X against the kernel trace: the dispatch after a packet
+// that records an event starts about 4.6 us after it (untimed launches follow each other with no
+// gap), so a sample is the kernel's execution plus that gap; kt_hold keeps the GPU from also
+// waiting for the host, which submits timed launches more slowly than they run.
 struct KernelTimer {
     ds_ctx* c;
     int id;
@@ -332,6 +335,13 @@ struct KernelTimer {
         if ((kt).a) hipExtLaunchKernelGGL(kern, grid, block, shm, stream, (kt).a, (kt).b, 0, __VA_ARGS__); \
         else hipLaunchKernelGGL(kern, grid, block, shm, stream, __VA_ARGS__);                    \
     } while (0)
+
+// At the start of a sampled substep: hold the stream for 0.5 ms so that the host queues the
+// substep's timed launches before the GPU reaches them.
+void kt_hold(ds_ctx* c, hipStream_t s)
+{
+    if (c->kt_on) hipLaunchKernelGGL(k_hold, dim3(1), dim3(64), 0, s, 50000l);
+}
 
 // After a synchronisation: add the elapsed time of every recorded pair.
 void kt_collect(ds_ctx* c)
@@ -843,6 +853,7 @@ int forward_tiled(ds_ctx* c, int nsteps)
         const int slot = table_slot(c, s);
         const float* in = c->d_state;
         c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
+        kt_hold(c, c->stream);
         if (s % R == 0) {
             // a checkpoint sort keeps its sorted state; a sort inside a segment goes through the
             // segment buffer, which nothing else uses during a forward rollout
@@ -900,6 +911,7 @@ void recompute_segment(ds_ctx* c, const Exec& x, int cp, int S, const SegSet& z)
         const int s = s0 + k;
         const int slot = table_slot(c, s);
         c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
+        kt_hold(c, x.stream);
         // the forward sorted again before step k + 1: this step writes the old order into tmp,
         // which the forward's permutation of that sort takes to the step's buffer
         const bool resort = k + 1 < L && (k + 1) % R == 0;
@@ -954,6 +966,7 @@ int adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
         const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)slot * c->NBK;
         const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)slot * c->NBK;
         c->kt_on = !c->capturing && c->kt_every > 0 && s % c->kt_every == 0;
+        kt_hold(c, c->stream);
         if (xs) {
             HIPCHK(c, hipStreamWaitEvent(c->stream3, c->ev_step, 0));
             KernelTimer kt(c, DS_K_P2G_ADJ_XGRAD, true);

@@ -9,6 +9,16 @@
 
 namespace ds {
 
+// Waits `ticks` of the constant-rate wall clock (100 MHz on gfx9), then ends: put in front of a
+// sampled substep's launches so that the host has queued them all before the GPU reaches them
+// (timed launches are slower to submit than to run, and a GPU waiting for the host would count
+// that wait into the samples).
+__global__ void k_hold(long ticks)
+{
+    const long t0 = (long)wall_clock64();
+    while ((long)wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(2);
+}
+
 __global__ void __launch_bounds__(256) k_fill_f4(float4* __restrict__ p, long n)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

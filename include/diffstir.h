@@ -197,7 +197,10 @@ int ds_synchronize(ds_ctx* ctx);
 int ds_last_timing(const ds_ctx* ctx, double* fwd_ms, double* bwd_ms);
 
 /* Per-kernel GPU time accumulated since the last reset by DS_OPT_KERNEL_TIMING sampling:
- * total_ms[DS_NKERNELS], launches[DS_NKERNELS]; reset != 0 clears the counters afterwards. */
+ * total_ms[DS_NKERNELS], launches[DS_NKERNELS]; reset != 0 clears the counters afterwards.  A
+ * sample is a HIP event pair on the launch (hipExtLaunchKernel), taken after a short hold of the
+ * stream so that the host has queued the substep; it includes the dispatch gap an event-recording
+ * packet causes (about 4.6 us on MI355X against the kernel trace). */
 int ds_kernel_timing(ds_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);
 
 /* FP64 3x3 signed SVD on the device, for tests: F[n][9] -> U[n][9], S[n][3], V[n][9]
