@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libdiffstir_hip.so")
 
 # the exported entry points of include/diffstir.h
 EXPORTS = (
-    "ds_create", "ds_destroy", "ds_set_option", "ds_last_error", "ds_set_particles", "ds_set_trajectory", "ds_set_params",
+    "ds_create", "ds_destroy", "ds_set_stream", "ds_set_option", "ds_last_error", "ds_set_particles", "ds_set_trajectory", "ds_set_params",
     "ds_reset", "ds_forward", "ds_current_step", "ds_get_forces", "ds_backward", "ds_read_liquid",
     "ds_read_stick", "ds_fallback_count", "ds_substep_noalloc", "ds_synchronize", "ds_last_timing", "ds_kernel_timing", "ds_svd3", "ds_svd3_values", "ds_svd3_backward",
 )
@@ -56,6 +56,14 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise LibraryNotBuilt(f"{LIB_PATH} is missing: build it with `python -m diffstir_amd.build`")
+    # A torch wheel carries its own HIP runtime under ROCm's soname (libamdhip64.so.7), and the
+    # first one loaded serves the whole process; loaded after this library, torch cannot bring up
+    # its device.  So torch is imported first when it is installed: torch tensors and streams and
+    # this library then share one runtime (a stream handle from torch is valid here).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     try:
         lib = C.CDLL(LIB_PATH)
     except OSError as e:
@@ -65,6 +73,7 @@ def load() -> C.CDLL:
         "ds_create": (C.c_int, [C.POINTER(SceneDesc), C.c_int32, I32P, I32P, I32P, C.c_int32, C.c_int32,
                                 C.POINTER(vp)]),
         "ds_destroy": (None, [vp]),
+        "ds_set_stream": (C.c_int, [vp, vp]),
         "ds_set_option": (C.c_int, [vp, C.c_int32, C.c_int32]),
         "ds_last_error": (C.c_char_p, [vp]),
         "ds_set_particles": (C.c_int, [vp, C.c_int32, F32P, F32P, F32P]),

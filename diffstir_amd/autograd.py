@@ -23,6 +23,9 @@ from .engine import Engine
 class StirRollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params: torch.Tensor, engine: Engine) -> torch.Tensor:
+        if params.is_cuda:
+            # the library's launches follow the work torch queued on its current stream
+            engine.set_stream(torch.cuda.current_stream(params.device).cuda_stream)
         p = params.detach().to("cpu", torch.float64).numpy()
         engine.set_params(p.reshape(engine.B, 4))
         engine.reset()

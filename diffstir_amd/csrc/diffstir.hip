@@ -157,7 +157,8 @@ struct ds_ctx {
     double kt_ms[DS_NKERNELS] = {0};
     int64_t kt_n[DS_NKERNELS] = {0};
 
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // where the context's work is queued: own_stream or the caller's
+    hipStream_t own_stream = nullptr;   // created with the context
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     std::string err;
@@ -464,9 +465,9 @@ void free_all(ds_ctx* c)
     }
     if (c->stream3) (void)hipStreamDestroy(c->stream3);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     c->ev0 = c->ev1 = nullptr;
-    c->stream = c->stream2 = c->stream3 = nullptr;
+    c->stream = c->own_stream = c->stream2 = c->stream3 = nullptr;
 }
 
 // Buffers that depend on the kernel family, allocated at the first ds_reset after a change.
@@ -1264,7 +1265,8 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
     ALLOC(c->d_traj, (size_t)c->B * c->T * 3);
     ALLOC(c->d_flag, 4);   // [0] failure bits, [2..3] the 64-bit fall-back counter
 #undef ALLOC
-    hipError_t e = hipStreamCreate(&c->stream);
+    hipError_t e = hipStreamCreate(&c->own_stream);
+    c->stream = c->own_stream;
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess)
@@ -1284,8 +1286,20 @@ void ds_destroy(ds_ctx* ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->own_stream && ctx->own_stream != ctx->stream) (void)hipStreamSynchronize(ctx->own_stream);
     free_all(ctx);
     delete ctx;
+}
+
+int ds_set_stream(ds_ctx* c, void* stream)
+{
+    if (!c) return DS_EINVAL;
+    hipStream_t next = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    if (next == c->stream) return DS_OK;
+    // the work queued so far comes before the caller's next work on the new stream
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->stream = next;
+    return DS_OK;
 }
 
 int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
@@ -1404,9 +1418,25 @@ int ds_set_particles(ds_ctx* c, int32_t inst, const float* xl, const float* xs, 
     if (!c) return DS_EINVAL;
     if (inst < 0 || inst >= c->B) return fail(c, DS_EINVAL, "instance index out of range");
     const float* src[3] = {xl, xs, xc};
+    static const char* names[3] = {"liquid", "stick", "container"};
+    // every stencil inside the grid, as the kernels require (their device flag would otherwise
+    // fail the first substep): the base cell int(x/dx - 0.5) in [0, n_grid - 3] per axis
+    const float inv_dx = c->dsc.inv_dx;
+    const int n = c->desc.n_grid;
     for (int sp = 0; sp < 3; sp++) {
         const Group& g = c->groups[inst * 3 + sp];
         if (g.count > 0 && !src[sp]) return fail(c, DS_EINVAL, "NULL positions for a non-empty species");
+        for (int i = 0; i < g.count; i++)
+            for (int a = 0; a < 3; a++) {
+                const float X = src[sp][3 * (size_t)i + a] * inv_dx;
+                const int b = std::isfinite(X) ? (int)(X - 0.5f) : -1;
+                if (b < 0 || b > n - 3)
+                    return fail(c, DS_EINVAL, std::string(names[sp]) + " particle " + std::to_string(i) +
+                                                  " is outside the grid's interior or not finite");
+            }
+    }
+    for (int sp = 0; sp < 3; sp++) {
+        const Group& g = c->groups[inst * 3 + sp];
         for (int i = 0; i < g.count; i++)
             for (int a = 0; a < 3; a++) c->x0_h[3 * ((size_t)g.offset + i) + a] = src[sp][3 * (size_t)i + a];
     }

@@ -174,6 +174,11 @@ class Engine:
         self._check(self.lib.ds_fallback_count(self.ctx, C.byref(n), int(bool(reset))))
         return int(n.value)
 
+    def set_stream(self, stream: Optional[int] = None):
+        """Queue the engine's work on a caller's HIP stream (a handle, e.g.
+        torch.cuda.current_stream().cuda_stream; None or 0: the engine's own stream)."""
+        self._check(self.lib.ds_set_stream(self.ctx, C.c_void_p(int(stream) if stream else None)))
+
     def set_kernel_timing(self, every: int):
         """Sample every N-th substep's kernels with HIP events (0: off)."""
         self._check(self.lib.ds_set_option(self.ctx, self.OPT_TIMING, int(every)))

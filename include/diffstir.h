@@ -10,9 +10,9 @@
  *
  * Conventions
  *  - Every call returns 0 on success or a DS_E* code; ds_last_error() gives the message.
- *  - Host pointers only (plain C arrays); the context owns all device memory.
- *  - Calls on one context are serialised on the context's HIP stream; a context is used by
- *    one host thread at a time.
+ *  - Host pointers (plain C arrays), except ds_substep_noalloc; the context owns all device memory.
+ *  - Calls on one context are serialised on the context's HIP stream (its own, or the caller's
+ *    after ds_set_stream); a context is used by one host thread at a time.
  *  - Forces: forces[b][s][3] for s = 0..max_steps-1, F(s) = m_stick/dt * sum_p dv[s,p]
  *    (simulation_functions.py:330-336, force_calculation_and_animation_module.py:16-19);
  *    row 0 is 0 because the stick starts with dv = 0.
@@ -124,6 +124,12 @@ int ds_create(const ds_scene_desc* desc, int32_t n_inst, const int32_t* n_liquid
               const int32_t* n_stick, const int32_t* n_container, int32_t max_steps,
               int32_t ckpt_every, ds_ctx** out);
 void ds_destroy(ds_ctx* ctx);
+/* Queue the context's work on `stream` (a hipStream_t) from the next call on; NULL: the context's
+ * own stream (a blocking stream, so it is ordered after work on the null stream).  The library's
+ * launches then follow what the caller queued on that stream (e.g. torch's current stream), and
+ * calls that return host results wait for that stream.  Waits for the previous stream's work.  No
+ * counterpart in the reference, whose kernels run on the Taichi runtime's own stream. */
+int ds_set_stream(ds_ctx* ctx, void* stream);
 /* Set an execution option (DS_OPT_*).  DS_OPT_KERNELS selects the kernel family: sorted
  * particles with LDS grid tiles (default) or one thread per particle with global atomics; it
  * takes effect at the next ds_reset. */
@@ -131,6 +137,8 @@ int ds_set_option(ds_ctx* ctx, int32_t option, int32_t value);
 const char* ds_last_error(const ds_ctx* ctx);
 
 /* Initial particle positions of one instance ([n][3] each; NULL allowed for an empty species).
+ * DS_EINVAL (nothing stored) when a position is not finite or its stencil leaves the grid: the
+ * base cell int(x/dx - 0.5) must lie in [0, n_grid - 3] on every axis.
  * Replaces initialize_objects' from_numpy uploads (simulation_functions.py:293-297). */
 int ds_set_particles(ds_ctx* ctx, int32_t inst, const float* x_liquid, const float* x_stick,
                      const float* x_container);

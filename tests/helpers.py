@@ -62,3 +62,24 @@ def gold_scene(name: str) -> tuple:
     assert np.allclose(chk, z["checksum"], rtol=0, atol=1e-6 * np.abs(z["checksum"]).max()), \
         f"scene generator drifted from the {name} fixture"
     return sc, z
+
+
+def reference_layout(root, frames: int, frames_per_clip: int = 10, force=None):
+    """A directory in the reference repository's layout (config/arguments.json, data/traj.npy,
+    data/force.npy) holding the first `frames` frames of the reference's data/traj.npy, for
+    diffstir_amd.reference_api.load()."""
+    import json
+    a = dict(S.REFERENCE_ARGUMENTS)
+    sim_keys = ("dt", "n_grid", "uniform_scaling_factor", "n_model_particles_generated",
+                "n_operator_model_particles_generated", "n_container_model_particles_generated")
+    doc = {"simulation_setting": {k: a[k] for k in sim_keys},
+           "scene_information": {k: v for k, v in a.items() if k not in sim_keys}}
+    doc["simulation_setting"]["frames_per_clip"] = frames_per_clip
+    os.makedirs(os.path.join(root, "config"), exist_ok=True)
+    os.makedirs(os.path.join(root, "data"), exist_ok=True)
+    with open(os.path.join(root, "config", "arguments.json"), "w") as f:
+        json.dump(doc, f)
+    np.save(os.path.join(root, "data", "traj.npy"), ref_traj()[:frames])
+    f = np.zeros(frames) if force is None else np.asarray(force, dtype=np.float64)[:frames]
+    np.save(os.path.join(root, "data", "force.npy"), f)
+    return str(root)

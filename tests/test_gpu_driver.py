@@ -113,6 +113,36 @@ def test_autograd_function_matches_engine_backward():
     np.testing.assert_allclose(p.grad.numpy(), d, rtol=1e-6)
 
 
+def test_model_selection_writes_the_reference_outputs(tmp_path):
+    """main.py:9-30 on a 30-substep reference scene: each model fitted (3 epochs of the first
+    clip), parameters_and_losses_result.json with the reference's keys and layout
+    (main.py:19-24), the best model replayed with its force profile and stick positions."""
+    import json
+    from diffstir_amd.main import model_selection
+    traj = ref_traj()[:31]
+    cfg = np.array([[[2.0, 0.5, 0.7, 3.0], [2.0, 0.5, 0.6, 2.0], [1.5, 2.0, 0.8, 1.0]],
+                    [[0.5, 0.1, 0.3, 0.5], [0.5, 0.1, 0.3, 0.5], [0.5, 0.5, 0.5, 0.0]],
+                    [[5.0, 1.0, 1.0, 5.0], [5.0, 1.0, 1.0, 5.0], [5.0, 5.0, 1.2, 2.0]]])
+    print("[gpu] model selection", flush=True)
+    r = model_selection(traj, cfg, epochs=1, out_dir=str(tmp_path), ckpt_every=10)
+    with open(os.path.join(tmp_path, "parameters_and_losses_result.json")) as f:
+        text = f.read()
+    d = json.loads(text)
+    assert list(d) == ["parameters", "losses"] and text.startswith('{\n    "parameters": [')
+    assert len(d["parameters"]) == 3 and all(len(p) == 4 for p in d["parameters"]) and len(d["losses"]) == 3
+    assert r["best_model"] == int(np.argmin(d["losses"]))
+    for i in range(3):
+        assert all(lo <= p <= hi for p, lo, hi in zip(d["parameters"][i][:3], cfg[1, i, :3], cfg[2, i, :3]))
+    fx = np.load(os.path.join(tmp_path, "force_profile_x.npy"))
+    sp = np.load(os.path.join(tmp_path, "stick_positions.npy"))
+    assert fx.shape == (31,) and sp.shape == (30, 3)
+    # the replay of the best model against the oracle (29 substeps, main.py:30)
+    best = r["best_model"]
+    sc = S.reference_scene(traj, seed=0, params=d["parameters"][best], model=best)
+    Fo = Oracle("f64").rollout(sc, nsteps=29)
+    assert force_err(np.stack([fx[:30], np.zeros(30), fx[:30]], 1)[:, [0]], Fo[:, [0]]) <= 1e-4
+
+
 def test_stale_rollout_adjoint_is_an_error():
     """An engine records one rollout.  Its adjoint after a second rollout (StirRollout applied
     twice before loss.backward()) or after new parameters raises instead of returning the
@@ -142,6 +172,28 @@ def test_stale_rollout_adjoint_is_an_error():
         assert np.all(np.isfinite(e.backward(g)))
         e.set_params(sc.params[None] * 1.2)       # unchanged values keep the tape valid
         assert np.all(np.isfinite(e.backward(g)))
+
+
+def test_engine_on_a_caller_stream_gives_the_same_result():
+    """ds_set_stream: the engine's work queued on a torch stream (and back on its own) gives the
+    same forces and gradient as the engine's own stream (same kernels, same order)."""
+    torch = pytest.importorskip("torch")
+    sc = S.tiny_scene(16, 12, seed=5)
+    print("[gpu] caller stream", flush=True)
+    out = []
+    side = torch.cuda.Stream()
+    for stream in (None, side.cuda_stream, None):
+        with Engine([sc], ckpt_every=4) as e:
+            e.set_stream(stream)
+            e.reset()
+            e.forward()
+            F = e.forces()
+            g = np.zeros_like(F)
+            g[0, 1:-1, 0] = F[0, 1:-1, 0]
+            out.append((F, e.backward(g)))
+    for F, d in out[1:]:
+        np.testing.assert_allclose(F, out[0][0], rtol=1e-12, atol=1e-12 * np.abs(out[0][0]).max())
+        np.testing.assert_allclose(d, out[0][1], rtol=1e-12)
 
 
 def test_batched_identification_reduces_the_loss():
@@ -178,9 +230,46 @@ def test_clip_hand_off_continues_from_the_last_state():
     np.testing.assert_array_equal(sim.engine.read_liquid(0)["x"], end)
     np.testing.assert_array_equal(sim.engine.read_stick(0)["x"], end_stick)
     sim.forward_steps()
-    assert np.all(np.isfinite(sim.engine.forces()))
+    F2 = sim.engine.forces()[0]
+    assert np.all(np.isfinite(F2))
+    # the second clip against the oracle rolled from the first clip's end state (gdm:131-138)
+    sc2 = dataclasses.replace(sc, x_liquid=end, x_stick=end_stick, traj=traj[10:20])
+    Fo = Oracle("f64").rollout(sc2)
+    assert force_err(F2, Fo) <= 1e-4
     with pytest.raises(ValueError):
         sim.hand_off_clip(traj[10:15])
     best, hists = fit_clips(sim, traj, target, 2, 1, [0.1, 0.0, 0.5, 0.0], [5.0, 20.0, 1.5, 1e4], n_fit=3)
     assert len(hists) == 2 and len(hists[0]["losses"]) == 3 and len(hists[1]["losses"]) == 1
     sim.close()
+
+
+def test_host_checks_and_the_device_failure_flag():
+    """Inputs the kernels cannot take are refused on the host before any launch (a NaN
+    parameter, a particle whose stencil leaves the grid); a rollout that blows up (a bulk modulus
+    far past the explicit time step's stability limit) ends with DS_ENONFINITE from the device
+    flag instead of NaN forces (the reference propagates NaN and its driver skips the update,
+    gradient_descent_module.py:18)."""
+    from diffstir_amd._lib import DiffStirError
+    sc = S.tiny_scene(16, 40, seed=1)
+    print("[gpu] host checks and the device failure flag", flush=True)
+    with Engine([sc], ckpt_every=5) as e:
+        with pytest.raises(DiffStirError) as ei:
+            e.set_params(np.array([[np.nan, 1.0, 1.0, 1e3]]))
+        assert ei.value.code == 1 and "non-finite" in str(ei.value)
+        for bad in (-0.05, 1.0 - 1.0 / 32, np.inf):
+            x = sc.x_liquid.copy()
+            x[7, 1] = bad
+            with pytest.raises(DiffStirError) as ei:
+                e.upload_scene(0, dataclasses.replace(sc, x_liquid=x))
+            assert ei.value.code == 1 and "liquid particle 7" in str(ei.value)
+        e.upload_scene(0, sc)
+        e.set_params(np.array([[1.0, 1.0, 1.0, 1e9]]))
+        e.reset()
+        with pytest.raises(DiffStirError) as ei:
+            e.forward()
+        assert ei.value.code == 5
+        # the context stays usable: a stable parameter set runs afterwards
+        e.set_params(sc.params[None])
+        e.reset()
+        e.forward()
+        assert np.all(np.isfinite(e.forces()))
