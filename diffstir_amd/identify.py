@@ -19,8 +19,9 @@ from .scene import Scene
 
 
 def synthetic_target(scene: Scene, truth: Sequence[float], noise: float = 0.01, seed: int = 0,
-                     ckpt_every: int = 50) -> np.ndarray:
-    """Forces of `scene` at parameters `truth` plus Gaussian noise of `noise` x the RMS force."""
+                     ckpt_every: int = 50, return_clean: bool = False):
+    """Forces of `scene` at parameters `truth` plus Gaussian noise of `noise` x the RMS force
+    (and the noiseless forces with return_clean)."""
     with Engine([scene], ckpt_every=ckpt_every) as e:
         e.set_params(np.asarray(truth, dtype=np.float64)[None])
         e.reset()
@@ -28,13 +29,17 @@ def synthetic_target(scene: Scene, truth: Sequence[float], noise: float = 0.01, 
         F = e.forces()[0]
     rms = np.sqrt(np.mean(F[2:] ** 2)) if F.shape[0] > 2 else 1.0
     rng = np.random.default_rng(seed)
-    return F + noise * rms * rng.normal(size=F.shape)
+    tgt = F + noise * rms * rng.normal(size=F.shape)
+    return (tgt, F) if return_clean else tgt
 
 
 def identify(scenes: Sequence[Scene], target: np.ndarray, init_params: np.ndarray, iterations: int = 100,
              lr: float = 1e-2, fit_mask: Sequence[bool] = (True, True, True, False), betas=(0.9, 0.999),
-             eps: float = 1e-8, ckpt_every: int = 50, callback=None, engine: Optional[Engine] = None) -> dict:
+             eps: float = 1e-8, ckpt_every: int = 50, callback=None, engine: Optional[Engine] = None,
+             lr_final: Optional[float] = None) -> dict:
     """Adam on log-parameters for B restarts (one instance each) against one target profile.
+    With `lr_final` the step size follows a cosine from `lr` at the first iteration to `lr_final`
+    at the last; otherwise it stays `lr`.
 
     Returns the per-iteration losses [iters, B], parameters [iters+1, B, 4] and the final
     parameters."""
@@ -63,7 +68,10 @@ def identify(scenes: Sequence[Scene], target: np.ndarray, init_params: np.ndarra
             v = betas[1] * v + (1 - betas[1]) * gphi * gphi
             mh = m / (1 - betas[0] ** (it + 1))
             vh = v / (1 - betas[1] ** (it + 1))
-            phi = phi - lr * mh / (np.sqrt(vh) + eps)
+            lr_it = lr
+            if lr_final is not None and iterations > 1:
+                lr_it = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * it / (iterations - 1)))
+            phi = phi - lr_it * mh / (np.sqrt(vh) + eps)
             theta = np.where(mask, np.exp(phi), theta)
             losses.append(L)
             hist.append(theta.copy())
@@ -87,18 +95,22 @@ def main(argv=None):
     ap.add_argument("--workload", default="ident16_n64")
     ap.add_argument("--iterations", type=int, default=100)
     ap.add_argument("--lr", type=float, default=1e-2)
+    ap.add_argument("--lr-final", type=float, default=None,
+                    help="cosine decay of the step size from --lr to this value over the iterations")
     ap.add_argument("--noise", type=float, default=0.01)
     ap.add_argument("--ckpt", type=int, default=50)
     ap.add_argument("--out", default="", help="also write the loss curve and parameter history (JSON)")
     a = ap.parse_args(argv)
     scenes = S.workload(a.workload)
     truth = np.asarray(scenes[0].params, dtype=np.float64)
-    tgt = synthetic_target(scenes[0], truth, noise=a.noise, ckpt_every=a.ckpt)
+    tgt, clean = synthetic_target(scenes[0], truth, noise=a.noise, ckpt_every=a.ckpt, return_clean=True)
+    from .loss import compute_loss
+    floor, _ = compute_loss(clean, tgt, scenes[0].cfg.loss_kind)
     rng = np.random.default_rng(11)
     inits = np.stack([S.hb_params_draw(rng, truth[3]) for _ in scenes])
     inits[:, :3] = np.maximum(inits[:, :3], 1e-2)
     t0 = time.perf_counter()
-    res = identify(scenes, tgt, inits, iterations=a.iterations, lr=a.lr, ckpt_every=a.ckpt,
+    res = identify(scenes, tgt, inits, iterations=a.iterations, lr=a.lr, ckpt_every=a.ckpt, lr_final=a.lr_final,
                    callback=lambda it, L, th: print(f"[identify] iteration {it}: loss min {np.min(L):.4g} "
                                                     f"median {np.median(L):.4g}", flush=True))
     el = time.perf_counter() - t0
@@ -107,7 +119,10 @@ def main(argv=None):
     summary = {"workload": a.workload, "restarts": len(scenes), "iterations": a.iterations,
                "substeps": scenes[0].max_steps - 1, "s_per_iteration": el / a.iterations,
                "gradient_evals_per_s": len(scenes) * a.iterations / el,
+               "lr": a.lr, "lr_final": a.lr_final, "noise_floor_loss": float(floor),
                "loss_first_min": float(np.min(res["losses"][0])), "loss_last_min": float(np.min(res["losses"][-1])),
+               "loss_first_median": float(np.median(res["losses"][0])),
+               "loss_last_median": float(np.median(res["losses"][-1])),
                "truth": truth[:3].tolist(), "best_restart": fin[:3].tolist(),
                "relative_error": (np.abs(fin[:3] - truth[:3]) / truth[:3]).tolist()}
     if a.out:
