@@ -39,6 +39,102 @@ __device__ __forceinline__ float species_mass(const DevScene& sc, int sp)
     return sp == LIQUID ? sc.m_liq : (sp == STICK ? sc.m_stick : sc.m_cont);
 }
 
+// ---------------------------------------------------------------------------- stencil sums
+//
+// The 27 weights factor, W(i,j,k) = wx_i wy_j wz_k, and the node offset o = (i,j,k) enters every
+// scatter and gather linearly (affine momenta, the C of G2P and their adjoints).  So the
+// routines below form the affine terms from their value at offset 0 by adding columns, and
+// reduce the gathers row by row (a row: the nodes k = 0, 1, 2 at fixed i, j) before one
+// weighted update per row, instead of forming every product per node.  The offsets are the
+// unrolled loop indices, so add_k costs nothing for k = 0 and one add for k = 1.
+
+// y + k x for a node offset k in {0, 1, 2}.  Both arms are formed before the choice so that
+// the choice is a select, which folds away for a constant k (fma(1, x, y) is x + y), rather
+// than a branch: a chain of such branches lets the optimiser sink the arithmetic of the whole
+// stencil past all 27 gathers, with every gathered value live at once.
+__device__ __forceinline__ float add_k(int k, float x, float y)
+{
+    const float r = fmaf((float)k, x, y);
+    return k == 0 ? y : r;
+}
+
+// Accumulators start at -0, the identity of IEEE addition, so the first add is exact and free.
+constexpr float kZero = -0.0f;
+
+// One row of a gathered vector g: t0 = sum_k wz_k g, t1 = sum_k k wz_k g.
+struct RowSums {
+    float t0[3] = {kZero, kZero, kZero};
+    float t1[3] = {kZero, kZero, kZero};
+    __device__ __forceinline__ void add(int k, float wz, float gx, float gy, float gz)
+    {
+        const float h[3] = {wz * gx, wz * gy, wz * gz};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            t0[a] += h[a];
+            t1[a] = add_k(k, h[a], t1[a]);
+        }
+    }
+};
+
+// Zeroth and first moments of a gathered vector over the stencil: m0 = sum W g,
+// m1[c] = sum W g o_c.
+struct Moments {
+    float m0[3] = {kZero, kZero, kZero};
+    float m1[3][3] = {{kZero, kZero, kZero}, {kZero, kZero, kZero}, {kZero, kZero, kZero}};
+    // row (i, j) with weight wij = wx_i wy_j
+    __device__ __forceinline__ void add_row(int i, int j, float wij, const RowSums& r)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float p = wij * r.t0[a];
+            m0[a] += p;
+            m1[0][a] = add_k(i, p, m1[0][a]);
+            m1[1][a] = add_k(j, p, m1[1][a]);
+            m1[2][a] = fmaf(wij, r.t1[a], m1[2][a]);
+        }
+    }
+    // sum W g_a (o_c - fx_c): the first moment about the particle
+    __device__ __forceinline__ float centred(int a, int c, const float fx[3]) const
+    {
+        return fmaf(-fx[c], m0[a], m1[c][a]);
+    }
+};
+
+// Columns and offset-0 value of an affine field y(o) = y0 + K (o - fx) s over the stencil:
+// col[c][a] = K[a][c] s, base = y0 - K fx s, so y(o) = base + sum_c o_c col[c].
+struct Affine {
+    float col[3][3];
+    float base[3];
+    __device__ __forceinline__ Affine(const float K[9], const float y0[3], float s, const float fx[3])
+    {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) col[c][a] = K[a * 3 + c] * s;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            base[a] = fmaf(-col[2][a], fx[2], fmaf(-col[1][a], fx[1], fmaf(-col[0][a], fx[0], y0[a])));
+    }
+    // value moved by offset k along axis c
+    __device__ __forceinline__ void step(int c, int k, const float in[3], float out[3]) const
+    {
+#pragma unroll
+        for (int a = 0; a < 3; a++) out[a] = add_k(k, col[c][a], in[a]);
+    }
+};
+
+// Row sums of a per-node scalar Wb for sum_nodes Wb dW, dW = (dwx wy wz, wx dwy wz, wx wy dwz):
+// a row holds P = sum_k wz_k Wb, Q = sum_k dwz_k Wb and adds (dwx wy P, wx dwy P, wx wy Q).
+struct WeightGrad {
+    float g[3] = {kZero, kZero, kZero};
+    __device__ __forceinline__ void add_row(float wx, float dwx, float wy, float dwy, float P, float Q)
+    {
+        g[0] = fmaf(dwx * wy, P, g[0]);
+        g[1] = fmaf(wx * dwy, P, g[1]);
+        g[2] = fmaf(wx * wy, Q, g[2]);
+    }
+};
+
 // ---------------------------------------------------------------------------- forward
 
 // What one particle scatters in P2G (simulation_functions.py:148-163): affine A, momentum m u
@@ -120,32 +216,26 @@ __device__ __forceinline__ void p2g_particle(const DevScene& sc, int sp, const I
 template <class Gather>
 __device__ __forceinline__ void g2p_gather(const DevScene& sc, const Stencil& s, Gather& acc, float nv[3], float nC[9])
 {
-#pragma unroll
-    for (int a = 0; a < 3; a++) nv[a] = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 9; a++) nC[a] = 0.0f;
+    Moments M;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const float d0 = (float)i - s.fx[0];
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const float d1 = (float)j - s.fx[1];
-            const float wij = s.w[0][i] * s.w[1][j];
+            RowSums r;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float d2 = (float)k - s.fx[2];
-                const float W = wij * s.w[2][k];
                 const float3 g = acc.get(s.b[0] + i, s.b[1] + j, s.b[2] + k);
-                const float wv[3] = {W * g.x, W * g.y, W * g.z};
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    nv[a] += wv[a];
-                    nC[a * 3 + 0] += sc.c4 * wv[a] * d0;
-                    nC[a * 3 + 1] += sc.c4 * wv[a] * d1;
-                    nC[a * 3 + 2] += sc.c4 * wv[a] * d2;
-                }
+                r.add(k, s.w[2][k], g.x, g.y, g.z);
             }
+            M.add_row(i, j, s.w[0][i] * s.w[1][j], r);
         }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nv[a] = M.m0[a];
+#pragma unroll
+        for (int c = 0; c < 3; c++) nC[a * 3 + c] = sc.c4 * M.centred(a, c, s.fx);
     }
 }
 
@@ -158,19 +248,22 @@ template <class Scatter>
 __device__ __forceinline__ void g2p_adj_scatter(const DevScene& sc, const Stencil& s, const float vt[3],
                                                 const float Cb[9], Scatter& acc)
 {
+    const Affine e(Cb, vt, sc.c4, s.fx);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
+        float bi[3];
+        e.step(0, i, e.base, bi);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
+            float bj[3];
+            e.step(1, j, bi, bj);
+            const float wij = s.w[0][i] * s.w[1][j];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
-                const float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
-                float e[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-                    e[a] = vt[a] + sc.c4 * (Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2]);
-                acc.add(s.b[0] + i, s.b[1] + j, s.b[2] + k, W * e[0], W * e[1], W * e[2]);
+                float q[3];
+                e.step(2, k, bj, q);
+                const float W = wij * s.w[2][k];
+                acc.add(s.b[0] + i, s.b[1] + j, s.b[2] + k, W * q[0], W * q[1], W * q[2]);
             }
         }
     }
@@ -184,32 +277,37 @@ template <bool kRolled = false, class Gather>
 __device__ __forceinline__ void g2p_adj_xgrad(const DevScene& sc, const Stencil& s, const float vt[3],
                                               const float Cb[9], Gather& gv, float xb[3])
 {
-    float fxb[3] = {0.0f, 0.0f, 0.0f};
+    // dL/dfx = sum Wb dW - c4 Cb^T sum W g, Wb = g . e with e(o) = vt + c4 Cb (o - fx)
+    const Affine e(Cb, vt, sc.c4, s.fx);
+    WeightGrad wg;
+    float m0[3] = {kZero, kZero, kZero};
     auto slice = [&](int i) {
         // one x-slice of nodes at a time (see p2g_adj_gather)
         asm volatile("" ::: "memory");
         const float w0 = i == 0 ? s.w[0][0] : (i == 1 ? s.w[0][1] : s.w[0][2]);
         const float dw0 = i == 0 ? s.dw[0][0] : (i == 1 ? s.dw[0][1] : s.dw[0][2]);
+        float bi[3];
+        e.step(0, i, e.base, bi);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
+            float bj[3];
+            e.step(1, j, bi, bj);
+            RowSums r;
+            float P = kZero, Q = kZero;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
-                const float W = w0 * s.w[1][j] * s.w[2][k];
-                float e[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-                    e[a] = vt[a] + sc.c4 * (Cb[a * 3 + 0] * d[0] + Cb[a * 3 + 1] * d[1] + Cb[a * 3 + 2] * d[2]);
+                float q[3];
+                e.step(2, k, bj, q);
                 const float3 g = gv.get(s.b[0] + i, s.b[1] + j, s.b[2] + k);
-                const float Wb = g.x * e[0] + g.y * e[1] + g.z * e[2];
-                const float dW[3] = {dw0 * s.w[1][j] * s.w[2][k], w0 * s.dw[1][j] * s.w[2][k],
-                                     w0 * s.w[1][j] * s.dw[2][k]};
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const float ctg = Cb[0 * 3 + c] * g.x + Cb[1 * 3 + c] * g.y + Cb[2 * 3 + c] * g.z;
-                    fxb[c] += Wb * dW[c] - sc.c4 * W * ctg;
-                }
+                const float Wb = g.x * q[0] + g.y * q[1] + g.z * q[2];
+                P = fmaf(s.w[2][k], Wb, P);
+                Q = fmaf(s.dw[2][k], Wb, Q);
+                r.add(k, s.w[2][k], g.x, g.y, g.z);
             }
+            wg.add_row(w0, dw0, s.w[1][j], s.dw[1][j], P, Q);
+            const float wij = w0 * s.w[1][j];
+#pragma unroll
+            for (int a = 0; a < 3; a++) m0[a] = fmaf(wij, r.t0[a], m0[a]);
         }
     };
     if (kRolled) {
@@ -219,8 +317,10 @@ __device__ __forceinline__ void g2p_adj_xgrad(const DevScene& sc, const Stencil&
 #pragma unroll
         for (int i = 0; i < 3; i++) slice(i);
     }
+    // c4 Cb^T sum W g with the columns of c4 Cb
 #pragma unroll
-    for (int a = 0; a < 3; a++) xb[a] = fxb[a] * sc.inv_dx;
+    for (int c = 0; c < 3; c++)
+        xb[c] = (wg.g[c] - (e.col[c][0] * m0[0] + e.col[c][1] * m0[1] + e.col[c][2] * m0[2])) * sc.inv_dx;
 }
 
 // Adjoint of G2P for one particle: scatter of dL/dv_grid and, for liquid (with_x), the
@@ -241,45 +341,50 @@ template <class Gather>
 __device__ __forceinline__ void p2g_adj_gather(const DevScene& sc, const Stencil& s, const float A[9], const float u[3],
                                                Gather& acc, float ub[3], float Ab[9], float fxb[3])
 {
+    // with g the momentum adjoint and mom(o) = m u + A (o - fx) dx: ub = m sum W g,
+    // Ab = dx sum W g (x) (o - fx), dL/dfx = sum Wb dW - dx A^T sum W g, Wb = g . mom + g_m m
     const float m = sc.m_liq;
-#pragma unroll
-    for (int a = 0; a < 3; a++) ub[a] = fxb[a] = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 9; a++) Ab[a] = 0.0f;
+    const float mu[3] = {m * u[0], m * u[1], m * u[2]};
+    const Affine mom(A, mu, sc.dx, s.fx);
+    Moments M;
+    WeightGrad wg;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         // one x-slice of nodes at a time: keeps the compiler from hoisting all 27 gathers
         // (and their registers) ahead of the arithmetic
         asm volatile("" ::: "memory");
+        float bi[3];
+        mom.step(0, i, mom.base, bi);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
+            float bj[3];
+            mom.step(1, j, bi, bj);
+            RowSums r;
+            float P = kZero, Q = kZero;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float dp[3] = {((float)i - s.fx[0]) * sc.dx, ((float)j - s.fx[1]) * sc.dx,
-                                     ((float)k - s.fx[2]) * sc.dx};
-                const float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
-                const float dW[3] = {s.dw[0][i] * s.w[1][j] * s.w[2][k], s.w[0][i] * s.dw[1][j] * s.w[2][k],
-                                     s.w[0][i] * s.w[1][j] * s.dw[2][k]};
+                float q[3];
+                mom.step(2, k, bj, q);
                 const float4 gq = acc.get(s.b[0] + i, s.b[1] + j, s.b[2] + k);
-                const float gg[3] = {gq.x, gq.y, gq.z};
-                float mom[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) mom[a] = m * u[a] + A[a * 3] * dp[0] + A[a * 3 + 1] * dp[1] + A[a * 3 + 2] * dp[2];
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    ub[a] += W * m * gg[a];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) Ab[a * 3 + c] += W * gg[a] * dp[c];
-                }
-                const float Wb = gg[0] * mom[0] + gg[1] * mom[1] + gg[2] * mom[2] + gq.w * m;
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const float atg = A[0 * 3 + c] * gg[0] + A[1 * 3 + c] * gg[1] + A[2 * 3 + c] * gg[2];
-                    fxb[c] += Wb * dW[c] - sc.dx * W * atg;
-                }
+                const float Wb = gq.x * q[0] + gq.y * q[1] + gq.z * q[2] + gq.w * m;
+                P = fmaf(s.w[2][k], Wb, P);
+                Q = fmaf(s.dw[2][k], Wb, Q);
+                r.add(k, s.w[2][k], gq.x, gq.y, gq.z);
             }
+            wg.add_row(s.w[0][i], s.dw[0][i], s.w[1][j], s.dw[1][j], P, Q);
+            M.add_row(i, j, s.w[0][i] * s.w[1][j], r);
         }
     }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        ub[a] = m * M.m0[a];
+#pragma unroll
+        for (int c = 0; c < 3; c++) Ab[a * 3 + c] = sc.dx * M.centred(a, c, s.fx);
+    }
+    // dx A^T sum W g with the columns of A dx (A itself is then dead after the set-up)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        fxb[c] = wg.g[c] - (mom.col[c][0] * M.m0[0] + mom.col[c][1] * M.m0[1] + mom.col[c][2] * M.m0[2]);
 }
 
 // Adjoint of P2G for one liquid particle, stress half: from Ab = dL/dA, back through
@@ -376,25 +481,24 @@ __device__ __forceinline__ void p2g_adj_liquid(const DevScene& sc, const InstPar
 template <class Gather>
 __device__ __forceinline__ void p2g_adj_rigid(const DevScene& sc, float m, const Stencil& s, Gather& acc, float Co[9])
 {
-    float Cb[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // Cb = dx sum W g (x) (o - fx), g the momentum adjoint
+    Moments M;
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++)
+        for (int j = 0; j < 3; j++) {
+            RowSums r;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float dp[3] = {((float)i - s.fx[0]) * sc.dx, ((float)j - s.fx[1]) * sc.dx,
-                                     ((float)k - s.fx[2]) * sc.dx};
-                const float W = s.w[0][i] * s.w[1][j] * s.w[2][k];
                 const float4 gq = acc.get(s.b[0] + i, s.b[1] + j, s.b[2] + k);
-                const float gg[3] = {gq.x, gq.y, gq.z};
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-#pragma unroll
-                    for (int c = 0; c < 3; c++) Cb[a * 3 + c] += W * gg[a] * dp[c];
+                r.add(k, s.w[2][k], gq.x, gq.y, gq.z);
             }
+            M.add_row(i, j, s.w[0][i] * s.w[1][j], r);
+        }
 #pragma unroll
-    for (int a = 0; a < 9; a++) Co[a] = m * Cb[a];
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) Co[a * 3 + c] = m * (sc.dx * M.centred(a, c, s.fx));
 }
 
 // ---------------------------------------------------------------------------- global accessors
