@@ -13,7 +13,9 @@ The options change how a substep runs on the device, never what it computes:
   ("packed": full wavefronts, lanes sharing a base served in successive passes).  With packed
   chunks the rows-per-wavefront choice does not apply.
 
-Later stages then refine the chosen set one option at a time:
+Later stages then refine the chosen set one option at a time (and a last pass re-checks the rows
+per wavefront and then the reduction split, whose best values depend on the options chosen after
+them):
 
 * ``sort_every`` -- sorting the particles into buckets more often than once per checkpoint
   interval (half or a quarter/fifth of it): where particles drift far within an interval (the
@@ -57,6 +59,9 @@ DEFAULT = {"particles_per_thread": 2, "gather_particles_per_thread": 1, "adjoint
 
 # options refined after the first stage, in order, with their default (first) values
 LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream", "g2p_split")
+# a last pass over the options whose best value depends most on the later ones: the rows per
+# wavefront (with the chosen sort interval and reduction split) and then the reduction split
+FINAL_STAGES = ("particles_per_thread", "reduce_parts")
 
 _CACHE: Dict[str, dict] = {}
 
@@ -88,6 +93,8 @@ def sort_candidates(ckpt_every: int) -> List[int]:
 def stage_candidates(key: str, ckpt_every: int, chosen: Optional[dict] = None) -> List:
     """Values of a later-stage option, its default first (`chosen`: the options chosen so far,
     which rule out values that would change nothing)."""
+    if key == "particles_per_thread":
+        return list(SPACE["particles_per_thread"])
     if key == "sort_every":
         return sort_candidates(ckpt_every)
     if key == "reduce_parts":
@@ -200,8 +207,12 @@ def tune(scenes, ckpt_every: int = 50, substeps: Optional[int] = None, space: Op
         best = pick((c, run(c)) for c in first)
         best_t = min(t for c, t in timings if c == best)
         # later stages: one option at a time, with the options chosen so far
-        for key in LATER_STAGES:
+        for key in LATER_STAGES + FINAL_STAGES:
+            if key == "particles_per_thread" and best.get("scatter_chunks") == "packed":
+                continue   # rows per wavefront do not apply to packed chunks
             vals = stage_candidates(key, ckpt_every, best)
+            if space and key in space:
+                vals = [v for v in vals if v in space[key]]
             stage = [(best, best_t)] + [(dict(best, **{key: v}), None) for v in vals if v != best[key]]
             stage = [(c, t if t is not None else run(c)) for c, t in stage]
             best = pick(stage)

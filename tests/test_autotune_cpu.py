@@ -65,8 +65,9 @@ def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
     n2 = len(A.sort_candidates(10)) - 1
     assert n2 == 2
     T = scenes[0].max_steps
-    # later stages: 2 sort intervals, 2 reduction splits, the stream, G2P split 2 (gppt 2 chosen)
-    assert len(calls) == nc + n2 + 4 and all(s == min(T - 1, 20) for _, s in calls)   # 2 checkpoint segments at most
+    # later stages: 2 sort intervals, 2 reduction splits, the stream, G2P split 2 (gppt 2 chosen);
+    # then the last pass: rows per wavefront 1 and 2, reduction splits 1 and 4
+    assert len(calls) == nc + n2 + 4 + 4 and all(s == min(T - 1, 20) for _, s in calls)   # 2 checkpoint segments at most
     # first stage: every kernel candidate with the later options at their defaults
     assert [c for c, _ in calls[:nc]] == [dict(c, sort_every=0, reduce_parts=1, xgrad_stream=False, g2p_split=1)
                                           for c in A.candidates()]
@@ -76,8 +77,10 @@ def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
                                                       g2p_split=1) for R in A.sort_candidates(10)[1:]]
     assert [c for c, _ in calls[nc + n2:nc + n2 + 2]] == [dict(want, sort_every=0, reduce_parts=p, xgrad_stream=False,
                                                                g2p_split=1) for p in (2, 4)]
-    assert calls[-2][0] == want2
-    assert calls[-1][0] == dict(want2, g2p_split=2)
+    assert calls[nc + n2 + 2][0] == want2
+    assert calls[nc + n2 + 3][0] == dict(want2, g2p_split=2)
+    assert [c for c, _ in calls[nc + n2 + 4:]] == [dict(want2, particles_per_thread=1), dict(want2, particles_per_thread=2),
+                                                  dict(want2, reduce_parts=1), dict(want2, reduce_parts=4)]
     # cached in memory and on disk by signature
     again = A.tune(scenes, ckpt_every=10, engine=eng, timer=lambda *a: pytest.fail("re-timed"))
     assert again["options"] == want2
@@ -87,6 +90,27 @@ def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
     from_disk = A.tune(scenes, ckpt_every=10, engine=eng, timer=lambda *a: pytest.fail("re-timed"),
                        cache_file=str(f))
     assert from_disk["options"] == want2
+    A.clear_cache()
+
+
+def test_last_pass_rechecks_rows_per_wavefront():
+    """A choice of rows per wavefront made with the later options at their defaults is revisited:
+    here 4 rows win at the default sort interval, 2 rows once the interval is shorter."""
+    A.clear_cache()
+    scenes = [S.tiny_scene(16, 20, seed=1)]
+
+    def timer(engine, cand, substeps, repeats):
+        if cand["gather_particles_per_thread"] != 1 or cand["adjoint_p2g"] != "fused" or cand["scatter_chunks"] != "rows":
+            return 9.0
+        t = {4: 1.0, 2: 1.1, 1: 1.2}[cand["particles_per_thread"]]
+        if cand["sort_every"]:
+            t -= 0.2 if cand["particles_per_thread"] == 4 else 0.4
+        return t + (0.0 if cand["reduce_parts"] == 1 else 0.05) + (0.05 if cand["xgrad_stream"] else 0.0)
+
+    res = A.tune(scenes, ckpt_every=10, engine=_FakeEngine(), timer=timer, warm_steps=0)
+    assert res["options"]["sort_every"] != 0
+    assert res["options"]["particles_per_thread"] == 2
+    assert res["options"]["reduce_parts"] == 1
     A.clear_cache()
 
 
