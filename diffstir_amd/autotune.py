@@ -29,7 +29,12 @@ them):
   extra tile load and stencil cost more than the overlap gains;
 * ``g2p_split`` -- the forward's G2P running each gather chunk as 2 or 4 workgroups: with 2 gather
   particles per thread chosen for the adjoint P2G, the forward's G2P can still run 256-particle
-  workgroups (only values up to the chosen particles per thread are tried).
+  workgroups (only values up to the chosen particles per thread are tried);
+* ``chunk_grid`` -- the launch grid of the chunk kernels: the library's default (16384
+  workgroups, of which those past the device-resident chunk count exit at once) or 2048 (a chunk
+  count above it loops).  With few chunks (one instance) the smaller grid leaves the two streams
+  of the overlapped adjoint fewer empty workgroups to dispatch; with many (large batches) the
+  larger grid lets the dispatcher spread chunks of uneven cost.
 
 Which choice is fastest depends on the workload's size and on how the particles cluster, so each
 candidate is timed on the workload itself: a short checkpointed forward + adjoint rollout
@@ -58,7 +63,7 @@ DEFAULT = {"particles_per_thread": 2, "gather_particles_per_thread": 1, "adjoint
            "scatter_chunks": "rows"}
 
 # options refined after the first stage, in order, with their default (first) values
-LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream", "g2p_split")
+LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream", "g2p_split", "chunk_grid")
 # a last pass over the options whose best value depends most on the later ones: the rows per
 # wavefront (with the chosen sort interval and reduction split) and then the reduction split
 FINAL_STAGES = ("particles_per_thread", "reduce_parts")
@@ -104,6 +109,8 @@ def stage_candidates(key: str, ckpt_every: int, chosen: Optional[dict] = None) -
     if key == "g2p_split":
         g = (chosen or {}).get("gather_particles_per_thread", 4)
         return [1] + [v for v in (2, 4) if v <= g]
+    if key == "chunk_grid":
+        return [0, 2048]
     raise KeyError(key)
 
 

@@ -91,6 +91,7 @@ class Engine:
         self.reduce_parts = 1
         self.xgrad_stream = False
         self.g2p_split = 1
+        self.chunk_grid = 0
         self.set_graph(graph)
         for b, s in enumerate(scenes):
             self.upload_scene(b, s)
@@ -147,6 +148,7 @@ class Engine:
             self.overlap = bool(overlap)
         if chunk_grid is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_CHUNK_GRID, int(chunk_grid)))
+            self.chunk_grid = int(chunk_grid)
         if sort_every is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_SORT_EVERY, int(sort_every)))
             self.sort_every = int(sort_every)
@@ -193,7 +195,8 @@ class Engine:
         return {"particles_per_thread": self.particles_per_thread,
                 "gather_particles_per_thread": self.gather_particles_per_thread, "adjoint_p2g": self.adjoint_p2g,
                 "scatter_chunks": self.scatter_chunks, "overlap": self.overlap, "sort_every": self.sort_every,
-                "reduce_parts": self.reduce_parts, "xgrad_stream": self.xgrad_stream, "g2p_split": self.g2p_split}
+                "reduce_parts": self.reduce_parts, "xgrad_stream": self.xgrad_stream, "g2p_split": self.g2p_split,
+                "chunk_grid": self.chunk_grid}
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):
