@@ -1028,8 +1028,8 @@ int adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
         }
         // first step after a sort: the sort's parameter-gradient partials, and the adjoint back
         // into the particle order before that sort (none before the very first one)
-        hipLaunchKernelGGL(k_thpart_reduce, dim3((unsigned)std::min<long>((c->maxgch + 255) / 256, 1024)), dim3(256),
-                           0, c->stream, gchunks, counts + 3, maxgch, c->d_thpart, c->d_thbar);
+        hipLaunchKernelGGL(k_thpart_reduce, dim3(c->B), dim3(256), 0, c->stream, gchunks, counts + 3, maxgch,
+                           c->d_thpart, c->d_thbar);
         if (k > 0) fill_zero_f64(c, c->d_thpart, 4L * c->maxgch);
         if (k > 0 || cp > 0) {
             hipLaunchKernelGGL(k_unsort_adj, dim3(nblocks(c->nreal)), dim3(256), 0, c->stream, c->d_adj, c->d_adj2,

@@ -551,4 +551,13 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// A workgroup's FP64 partial of a sum that workgroups add with atomics, rounded to a multiple of
+// 2^-32.  Sums of such values stay multiples of 2^-32, and below 2^21 in magnitude every one of
+// them is exact in FP64 (53-bit significands), so the atomic adds give the same bits in any order.
+// The rounding moves a partial by at most 2^-33.
+__device__ __forceinline__ double exact_grain(double x)
+{
+    return rint(x * 4294967296.0) * (1.0 / 4294967296.0);
+}
+
 }  // namespace ds

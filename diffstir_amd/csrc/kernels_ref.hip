@@ -186,9 +186,9 @@ __global__ void __launch_bounds__(256) k_g2p_ref(const float* __restrict__ in, f
         double s0 = wave_sum(dv[0]), s1 = wave_sum(dv[1]), s2 = wave_sum(dv[2]);
         if ((threadIdx.x & 63) == 0) {
             double* d = dvsum + ((long)groups[gw].inst * T + f + 1) * 3;
-            atomicAdd(d + 0, s0);
-            atomicAdd(d + 1, s1);
-            atomicAdd(d + 2, s2);
+            atomicAdd(d + 0, exact_grain(s0));
+            atomicAdd(d + 1, exact_grain(s1));
+            atomicAdd(d + 2, exact_grain(s2));
         }
     }
 }

@@ -632,7 +632,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tiled(const float* __restrict__ in, 
             if (threadIdx.x < 3) {
                 double acc2 = 0.0;
                 for (int k = 0; k < WAVES; k++) acc2 += red[k][threadIdx.x];
-                atomicAdd(dvsum + ((long)ch.inst * T + f + 1) * 3 + threadIdx.x, acc2);
+                atomicAdd(dvsum + ((long)ch.inst * T + f + 1) * 3 + threadIdx.x, exact_grain(acc2));
             }
         }
         __syncthreads();
@@ -1094,45 +1094,44 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_stress(const float* __restrict__
     }
 }
 
-// The per-chunk parameter-gradient partials of a segment into the per-instance gradient: a
-// workgroup sums the partials of its chunks with one atomic per parameter when they all belong
-// to one instance (the usual case: chunks are ordered by instance), else one per chunk.
+// The per-chunk FP64 parameter-gradient partials of one sort's gather chunks, added into thbar:
+// one workgroup per instance, in a fixed order (thread t takes the instance's chunks t, t + 256,
+// ..., then the wave sums and the wavefronts in order), so the sum has the same bits in every run.
+// An instance's chunks are contiguous in the table (bucket order, instance first); the workgroup
+// finds them by binary search over the chunks' instance numbers.
 __global__ void __launch_bounds__(256) k_thpart_reduce(const GChunk* __restrict__ chunks,
                                                        const int* __restrict__ nchunks_p, int maxchunks,
                                                        const double* __restrict__ thpart, double* __restrict__ thbar)
 {
     __shared__ double red[WAVES][4];
-    __shared__ int inst0;
     const int nchunks = min(*nchunks_p, maxchunks);
-    for (int base = blockIdx.x * blockDim.x; base < nchunks; base += gridDim.x * blockDim.x) {
-        const int c = base + threadIdx.x;
-        if (threadIdx.x == 0) inst0 = chunks[base].inst;
-        __syncthreads();
-        const bool ok = c < nchunks;
-        const int inst = ok ? chunks[c].inst : inst0;
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-        if (ok) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) v[k] = thpart[4 * (long)c + k];
+    const int inst = blockIdx.x;
+    auto first_at_least = [&](int key) {
+        int lo = 0, hi = nchunks;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (chunks[mid].inst < key) lo = mid + 1;
+            else hi = mid;
         }
-        if (__syncthreads_and(inst == inst0)) {
-            const int w = threadIdx.x >> 6;
+        return lo;
+    };
+    const int c0 = first_at_least(inst), c1 = first_at_least(inst + 1);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const double r = wave_sum(v[k]);
-                if ((threadIdx.x & 63) == 0) red[w][k] = r;
-            }
-            __syncthreads();
-            if (threadIdx.x < 4) {
-                double a = 0.0;
-                for (int k = 0; k < WAVES; k++) a += red[k][threadIdx.x];
-                atomicAdd(thbar + 4 * inst0 + threadIdx.x, a);
-            }
-        } else if (ok) {
+        for (int k = 0; k < 4; k++) v[k] += thpart[4 * (long)c + k];
+    }
+    const int w = threadIdx.x >> 6;
 #pragma unroll
-            for (int k = 0; k < 4; k++) atomicAdd(thbar + 4 * inst + k, v[k]);
-        }
-        __syncthreads();
+    for (int k = 0; k < 4; k++) {
+        const double r = wave_sum(v[k]);
+        if ((threadIdx.x & 63) == 0) red[w][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double a = 0.0;
+        for (int k = 0; k < WAVES; k++) a += red[k][threadIdx.x];
+        thbar[4 * inst + threadIdx.x] += a;
     }
 }
 
