@@ -118,8 +118,8 @@ struct ds_ctx {
     GChunk* d_gchunks = nullptr;  // ncp x maxgch chunks of the gather-only kernels
     int* d_bgch = nullptr;        // ncp x NBK gather chunks per bucket
     int* d_gstart = nullptr;      // ncp x NBK first gather chunk of each bucket
-    int* d_cov = nullptr;         // ncp x B x NB^3 node blocks reached by a chunk tile
-    int* d_fbflag = nullptr;      // B x NB^3 node blocks marked by a fall-back scatter this step
+    int* d_cov = nullptr;         // ncp x B x NR^3 reduction blocks reached by a chunk tile
+    int* d_fbflag = nullptr;      // B x NR^3 reduction blocks marked by a fall-back scatter this step
     double* d_j = nullptr;        // P: J of each liquid particle for the next P2G
     double* d_seg_mj = nullptr;   // K x 10 x P: J and its SVD backward factor per step of a segment
     float* d_gadj = nullptr;      // 18 x P: gathered quantities of the adjoint P2G
@@ -145,6 +145,7 @@ struct ds_ctx {
     int* d_rstart = nullptr;      // first row of each bucket
     int* d_src = nullptr;         // ncp x P
     int NB = 0, NBK = 0;
+    int NR = 0;                   // reduction blocks per axis (n / 4 + 1: block r owns nodes [4r - 1, 4r + 3))
     long maxch = 0, maxgch = 0;
     int* d_counts = nullptr;      // ncp x 4: sort totals of each checkpoint (particles, chunks, rows,
                                   // gather chunks), read by the kernels; the host never reads them back
@@ -492,6 +493,7 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_pbar, G);
     } else {
         c->NB = (c->desc.n_grid + TB - 1) / TB;
+        c->NR = c->desc.n_grid / TB + 1;
         c->NBK = c->B * 3 * c->NB * c->NB * c->NB;
         c->E = c->K / (c->sort_every > 0 ? c->sort_every : c->K);
         c->nslot = c->ncp * c->E;
@@ -522,8 +524,8 @@ int alloc_mode(ds_ctx* c)
         MALLOC(c->d_bgch, (size_t)c->NBK * c->nslot);
         MALLOC(c->d_gstart, (size_t)c->NBK * c->nslot);
         MALLOC(c->d_gchunks, (size_t)c->maxgch * c->nslot);
-        MALLOC(c->d_cov, (size_t)c->B * c->NB * c->NB * c->NB * c->nslot);
-        MALLOC(c->d_fbflag, (size_t)c->B * c->NB * c->NB * c->NB);
+        MALLOC(c->d_cov, (size_t)c->B * c->NR * c->NR * c->NR * c->nslot);
+        MALLOC(c->d_fbflag, (size_t)c->B * c->NR * c->NR * c->NR);
         MALLOC(c->d_j, c->P);
         MALLOC(c->d_seg_mj, (size_t)10 * c->P * c->K);
         MALLOC(c->d_gadj, (size_t)18 * c->P);
@@ -543,7 +545,7 @@ int alloc_mode(ds_ctx* c)
         // staging slots for either chunk set at its largest
         c->staging_cap = std::max(c->maxch, c->maxgch);
         MALLOC(c->d_staging, (size_t)c->staging_cap * TNODES);
-        fill_i32(c, c->d_fbflag, (long)c->B * c->NB * c->NB * c->NB, 0);
+        fill_i32(c, c->d_fbflag, (long)c->B * c->NR * c->NR * c->NR, 0);
         fill_zero_f4(c, c->d_pfb, G);
         fill_zero_f4(c, c->d_vbfb, G);
     }
@@ -583,10 +585,10 @@ int alloc_overlap(ds_ctx* c)
     MALLOC(c->d_seg_aff_b, (size_t)9 * c->P * c->K);
     MALLOC(c->d_seg_mj_b, (size_t)10 * c->P * c->K);
     MALLOC(c->d_staging_r, (size_t)c->staging_cap * TNODES);
-    MALLOC(c->d_fbflag_r, (size_t)c->B * c->NB * c->NB * c->NB);
+    MALLOC(c->d_fbflag_r, (size_t)c->B * c->NR * c->NR * c->NR);
     if (c->E > 1) MALLOC(c->d_seg_tmp_b, SF);
 #undef MALLOC
-    fill_i32(c, c->d_fbflag_r, (long)c->B * c->NB * c->NB * c->NB, 0);
+    fill_i32(c, c->d_fbflag_r, (long)c->B * c->NR * c->NR * c->NR, 0);
     if (!c->stream2) HIPCHK(c, hipStreamCreate(&c->stream2));
     for (hipEvent_t* e : {&c->ev_fork, &c->ev_rec[0], &c->ev_rec[1], &c->ev_adj[0], &c->ev_adj[1]})
         if (!*e) HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
@@ -697,12 +699,12 @@ int sort_into(ds_ctx* c, int slot, float* dst)
                        c->d_bstart, c->d_clist, c->P);
     hipLaunchKernelGGL(k_rank_fix, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_key, c->d_cnt, c->d_celloff,
                        c->d_bstart, c->d_clist, c->d_pid, c->d_rank, c->P);
-    const long nbl = (long)c->B * c->NB * c->NB * c->NB;
+    const long nbl = (long)c->B * c->NR * c->NR * c->NR;
     int* cov = c->d_cov + (size_t)cp * nbl;
     fill_i32(c, cov, nbl, 0);
     hipLaunchKernelGGL(k_bucket_layout, dim3(nblocks((long)c->NBK * 64)), dim3(256), 0, c->stream, c->d_cnt,
                        c->d_celloff, c->d_bstart, c->d_rstart, cstart, gstart, c->d_rrpos, rows, chunks, gchunks,
-                       cov, c->NB, c->NBK, rpc, gch);
+                       cov, c->NB, c->NR, c->NBK, rpc, gch);
     fill_i32(c, c->d_meta2, c->P, -1);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(c->P)), dim3(256), 0, c->stream, c->d_state, dst, c->d_key,
                        c->d_rank, c->d_celloff, c->d_bstart, c->d_rrpos, c->d_meta, c->d_meta2, c->d_pid, c->d_pid2,
@@ -789,19 +791,19 @@ void p2g_reduce_tiled(ds_ctx* c, const Exec& x, const float* in, float* fout, in
         if (c->packed)
             DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_p2g_tiled, GChunk, chunk_grid(c, c->maxgch), dim3(WG), 0, x.stream,
                                in, fout, aout, c->d_gchunks + (size_t)cp * c->maxgch, nullptr, counts + 3,
-                               (int)c->maxgch, c->d_params, x.staging, c->d_pfb, x.fbflag, c->NB, c->d_j, c->dsc,
+                               (int)c->maxgch, c->d_params, x.staging, c->d_pfb, x.fbflag, c->NR, c->d_j, c->dsc,
                                c->P, c->d_flag);
         else
             DS_PPT_DISPATCH2_T(kt, c->ppt_sorted, k_p2g_tiled, Chunk, chunk_grid(c, c->maxch), dim3(WG), 0, x.stream, in,
                                fout, aout, c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1),
-                               counts + 1, (int)c->maxch, c->d_params, x.staging, c->d_pfb, x.fbflag, c->NB,
+                               counts + 1, (int)c->maxch, c->d_params, x.staging, c->d_pfb, x.fbflag, c->NR,
                                c->d_j, c->dsc, c->P, c->d_flag);
     }
-    const int nb3 = c->NB * c->NB * c->NB;
+    const int nr3 = c->NR * c->NR * c->NR;
     KernelTimer kt(c, DS_K_GRID_REDUCE, true);
-    const int* cov = c->d_cov + (size_t)cp * c->B * nb3;
-    DS_PARTS_DISPATCH_T(kt, c->rparts, k_grid_reduce, dim3(c->B * nb3), x.stream, x.staging, bnch, cstart, cov,
-                        c->d_pfb, x.fbflag, pout, c->d_vgrid, c->dsc, c->NB);
+    const int* cov = c->d_cov + (size_t)cp * c->B * nr3;
+    DS_PARTS_DISPATCH_T(kt, c->rparts, k_grid_reduce, dim3(c->B * nr3), x.stream, x.staging, bnch, cstart, cov,
+                        c->d_pfb, x.fbflag, pout, c->d_vgrid, c->dsc, c->NB, c->NR);
 }
 
 void g2p_tiled(ds_ctx* c, const Exec& x, const float* in, float* out, int cp, int f, bool record,
@@ -938,7 +940,7 @@ int adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
 {
     const long SF = c->stateFloats();
     const long G = c->gridNodes() * c->B;
-    const int nb3 = c->NB * c->NB * c->NB;
+    const int nr3 = c->NR * c->NR * c->NR;
     const dim3 rgrid = chunk_grid(c, c->maxch), ggrid = chunk_grid(c, c->maxgch);
     const int maxch = (int)c->maxch, maxgch = (int)c->maxgch;
     const int R = sort_interval(c);
@@ -980,17 +982,17 @@ int adjoint_segment(ds_ctx* c, int cp, int S, const SegSet& z)
             if (c->packed)
                 DS_PPT_DISPATCH2_T(kt, c->gppt_sorted, k_g2p_adj_tiled, GChunk, ggrid, dim3(WG), 0, c->stream,
                                    st_of(k), c->d_adj, gchunks, nullptr, counts + 3, maxgch, c->d_dvbar, c->T, s,
-                                   c->d_staging, c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
+                                   c->d_staging, c->d_vbfb, c->d_fbflag, c->NR, c->dsc, c->P);
             else
                 DS_PPT_DISPATCH2_T(kt, c->ppt_sorted, k_g2p_adj_tiled, Chunk, rgrid, dim3(WG), 0, c->stream, st_of(k),
                                    c->d_adj, chunks, rows, counts + 1, maxch, c->d_dvbar, c->T, s, c->d_staging,
-                                   c->d_vbfb, c->d_fbflag, c->NB, c->dsc, c->P);
+                                   c->d_vbfb, c->d_fbflag, c->NR, c->dsc, c->P);
         }
         {
             KernelTimer kt(c, DS_K_VBAR_REDUCE, true);
-            DS_PARTS_DISPATCH_T(kt, c->rparts, k_vbar_reduce, dim3(c->B * nb3), c->stream, c->d_staging, bnch,
-                                cstart, c->d_cov + (size_t)slot * c->B * nb3, c->d_vbfb, c->d_fbflag, grid_of(k),
-                                c->d_pbar2, c->dsc, c->NB);
+            DS_PARTS_DISPATCH_T(kt, c->rparts, k_vbar_reduce, dim3(c->B * nr3), c->stream, c->d_staging, bnch,
+                                cstart, c->d_cov + (size_t)slot * c->B * nr3, c->d_vbfb, c->d_fbflag, grid_of(k),
+                                c->d_pbar2, c->dsc, c->NB, c->NR);
         }
         if (xs) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_xg, 0));
         if (c->adj_split) {

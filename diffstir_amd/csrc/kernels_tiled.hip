@@ -134,16 +134,22 @@ __device__ __forceinline__ Stencil opaque(const Stencil& s)
     return t;
 }
 
-// Marks the node blocks a stencil touches (its corner nodes' blocks) in the per-step fall-back
-// flags, so the reduction visits them even where no chunk tile covers them.
-__device__ __forceinline__ void mark_fallback_blocks(int* __restrict__ fbflag, int inst, const Stencil& s, int NB)
+// Reduction blocks: the per-node reductions run on 4x4x4 node blocks shifted by one node, block
+// r owning nodes [4r - 1, 4r + 3) per axis (NR = n / 4 + 1 blocks per axis).  Every node of block
+// r then lies in the tiles of the same buckets, r - 1 and r per axis, so the block's nodes share
+// one list of covering chunk slots.
+__device__ __forceinline__ int rblock(int node) { return (node + 1) / TB; }
+
+// Marks the reduction blocks a stencil touches (its corner nodes' blocks) in the per-step
+// fall-back flags, so the reduction visits them even where no chunk tile covers them.
+__device__ __forceinline__ void mark_fallback_blocks(int* __restrict__ fbflag, int inst, const Stencil& s, int NR)
 {
-    const int nb3 = NB * NB * NB;
+    const int nr3 = NR * NR * NR;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        const int bx = (s.b[0] + ((q >> 2) & 1) * 2) / TB, by = (s.b[1] + ((q >> 1) & 1) * 2) / TB,
-                  bz = (s.b[2] + (q & 1) * 2) / TB;
-        atomicOr(fbflag + (long)inst * nb3 + (bx * NB + by) * NB + bz, 1);
+        const int rx = rblock(s.b[0] + ((q >> 2) & 1) * 2), ry = rblock(s.b[1] + ((q >> 1) & 1) * 2),
+                  rz = rblock(s.b[2] + (q & 1) * 2);
+        atomicOr(fbflag + (long)inst * nr3 + (rx * NR + ry) * NR + rz, 1);
     }
 }
 
@@ -280,7 +286,7 @@ __global__ void __launch_bounds__(WG, P2GMinWaves<CH>::value) k_p2g_tiled(const 
                                                   const int* __restrict__ rows,
                                                   const int* __restrict__ nchunks_p, int maxchunks,
                                                   const double* __restrict__ params, float4* __restrict__ staging,
-                                                  float4* __restrict__ fallback, int* __restrict__ fbflag, int NB,
+                                                  float4* __restrict__ fallback, int* __restrict__ fbflag, int NR,
                                                   const double* __restrict__ jin, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
 {
@@ -329,7 +335,7 @@ __global__ void __launch_bounds__(WG, P2GMinWaves<CH>::value) k_p2g_tiled(const 
                 }
                 it = stencil_in_tile(s, x0, y0, z0, sc.tile_win);
                 if (!it) {
-                    mark_fallback_blocks(fbflag, ch.inst, s, NB);
+                    mark_fallback_blocks(fbflag, ch.inst, s, NR);
                     if (sc.count_fallback) atomicAdd(reinterpret_cast<unsigned long long*>(flag + 2), 1ull);
                     ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
                     p2g_scatter(sc, s, pl, g);
@@ -362,16 +368,16 @@ __global__ void __launch_bounds__(WG, P2GMinWaves<CH>::value) k_p2g_tiled(const 
     }
 }
 
-// The chunk counts and first chunks of the 27 buckets around node block (bx, by, bz), all
-// three species, into LDS (index sp * 27 + (dx+1) * 9 + (dy+1) * 3 + (dz+1); 0 chunks outside
-// the grid).
-__device__ __forceinline__ void load_neighbour_buckets(int* s_nc, int* s_cs, const int* __restrict__ bnch,
-                                                       const int* __restrict__ cstart, int inst, int bx, int by,
-                                                       int bz, int NB)
+// The chunk counts and first chunks of the 8 buckets covering reduction block (rx, ry, rz) (buckets
+// rx - 1 + dx, dx in {0, 1}, per axis), all three species, into LDS (index sp * 8 + dx * 4 + dy * 2
+// + dz; 0 chunks outside the grid).
+__device__ __forceinline__ void load_covering_buckets(int* s_nc, int* s_cs, const int* __restrict__ bnch,
+                                                      const int* __restrict__ cstart, int inst, int rx, int ry,
+                                                      int rz, int NB)
 {
-    for (int e = threadIdx.x; e < 3 * 27; e += blockDim.x) {
-        const int sp = e / 27, r = e % 27;
-        const int cx = bx + r / 9 - 1, cy = by + (r / 3) % 3 - 1, cz = bz + r % 3 - 1;
+    for (int e = threadIdx.x; e < 24; e += blockDim.x) {
+        const int sp = e >> 3;
+        const int cx = rx - 1 + ((e >> 2) & 1), cy = ry - 1 + ((e >> 1) & 1), cz = rz - 1 + (e & 1);
         int nc = 0, cs = 0;
         if (cx >= 0 && cy >= 0 && cz >= 0 && cx < NB && cy < NB && cz < NB) {
             const int bucket = ((inst * 3 + sp) * NB + cx) * NB * NB + cy * NB + cz;
@@ -383,85 +389,81 @@ __device__ __forceinline__ void load_neighbour_buckets(int* s_nc, int* s_cs, con
     }
 }
 
-// Sum, in a fixed order, the staging entries of node (local lx, ly, lz of its 4x4x4 node
-// block) from every chunk whose tile covers it.  Per axis the covering buckets are b-1 and b
-// for local nodes 0..2, b and b+1 for local node 3; the node's index in the tile of bucket
-// b + d is local - 4d + 1.  The 24 (species, covering bucket) pairs are visited together:
-// round k loads the k-th chunk slot of every pair at once, so a round costs one memory latency.
-// This call takes rounds k0, k0 + kstep, ... (the parts of a split reduction).  kReread: read
-// each pair's table entries from LDS again in every round instead of holding 48 values in
-// registers across the rounds (the adjoint reduction: 126 instead of 143 registers, a fourth
-// wavefront per SIMD, about half the time on the 32-instance batch; the forward reduction's
-// other live values keep it at 3 wavefronts either way, so it keeps the registers).
-template <bool kReread>
-__device__ __forceinline__ void sum_covering_slots(float4& acc, const float4* __restrict__ staging, const int* s_nc,
-                                                   const int* s_cs, int lx, int ly, int lz, int ncomp, int k0 = 0,
-                                                   int kstep = 1)
+// Sum, in a fixed order, the staging entries of node (lx, ly, lz) of a reduction block from every
+// chunk whose tile covers it.  The node lies in the tiles of buckets r - 1 and r per axis, at index
+// l + 4 - 4d in the tile of bucket r - 1 + d.  The covering entries form one list, the same for
+// every node of the block: the 24 (species, covering bucket) pairs in order, and within a pair
+// its chunks in order.  The walk over the list is uniform across the wavefront (scalar); it goes
+// in batches of KB entries whose addresses are formed first and whose loads are then all in
+// flight at once, so a batch costs one memory latency, and the 64 lanes of a load read 64 nodes
+// of one chunk slot.  This call takes list entries part, part + nparts, ... (the parts of a split
+// reduction).
+template <int KB>
+__device__ __forceinline__ void sum_covering_list(float4& acc, const float4* __restrict__ staging, const int* s_nc,
+                                                  const int* s_cs, bool inb, int lx, int ly, int lz, int ncomp,
+                                                  int part, int nparts)
 {
-    const int ox = lx < 3 ? -1 : 0, oy = ly < 3 ? -1 : 0, oz = lz < 3 ? -1 : 0;
-    // pair q's bucket entry in the LDS tables and the node's index in its chunks' slots
-    auto pair = [&](int q, int& e) {
-        const int sp = q >> 3, dx = ox + ((q >> 2) & 1), dy = oy + ((q >> 1) & 1), dz = oz + (q & 1);
-        e = ((lx - 4 * dx + 1) * TS + (ly - 4 * dy + 1)) * TS + (lz - 4 * dz + 1);
-        return sp * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+    // the node's index in the slot of pair q's bucket
+    auto node_in_tile = [&](int q) {
+        const int dx = (q >> 2) & 1, dy = (q >> 1) & 1, dz = q & 1;
+        return ((lx + 4 - 4 * dx) * TS + (ly + 4 - 4 * dy)) * TS + (lz + 4 - 4 * dz);
     };
-    int first[24], nc[24];
-    int kmax = 0;
-#pragma unroll
-    for (int q = 0; q < 24; q++) {
-        int e;
-        const int r = pair(q, e);
-        nc[q] = s_nc[r];
-        if (!kReread) first[q] = s_cs[r] * TNODES + e;
-        kmax = max(kmax, nc[q]);
-    }
-    for (int k = k0; k < kmax; k += kstep) {
-        float4 v[24];
-        if (kReread) asm volatile("" ::: "memory");   // keeps the table reads inside the round
-#pragma unroll
-        for (int q = 0; q < 24; q++) {
-            int n = nc[q], f = 0;
-            if (kReread) {
-                int e;
-                const int r = pair(q, e);
-                n = s_nc[r];
-                f = s_cs[r] * TNODES + e;
-            } else {
-                f = first[q];
+    // walk position: chunk k of pair q (q == 24: past the end); n, cs: pair q's chunk count and
+    // first chunk
+    int q = 0, k = part;
+    int n = __builtin_amdgcn_readfirstlane(s_nc[0]), cs = __builtin_amdgcn_readfirstlane(s_cs[0]);
+    auto settle = [&]() {
+        while (q < 24 && k >= n) {
+            k -= n;
+            if (++q < 24) {
+                n = __builtin_amdgcn_readfirstlane(s_nc[q]);
+                cs = __builtin_amdgcn_readfirstlane(s_cs[q]);
             }
-            v[q] = k < n ? staging[f + k * TNODES] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
+    };
+    settle();
+    while (q < 24) {
+        int a[KB];
 #pragma unroll
-        for (int q = 0; q < 24; q++) {
-            acc.x += v[q].x;
-            acc.y += v[q].y;
-            acc.z += v[q].z;
-            if (ncomp == 4) acc.w += v[q].w;
+        for (int j = 0; j < KB; j++) {
+            a[j] = q < 24 && inb ? (cs + k) * TNODES + node_in_tile(q) : -1;
+            k += nparts;
+            settle();
+        }
+        float4 v[KB];
+#pragma unroll
+        for (int j = 0; j < KB; j++) v[j] = a[j] >= 0 ? staging[a[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int j = 0; j < KB; j++) {
+            acc.x += v[j].x;
+            acc.y += v[j].y;
+            acc.z += v[j].z;
+            if (ncomp == 4) acc.w += v[j].w;
         }
     }
 }
 
-// The staged sum of one node of a node block in NPART parts: wavefront w of the workgroup takes
-// rounds w, w + NPART, ... of the covering slots (so a block whose buckets hold many chunks
-// waits for ceil(rounds / NPART) memory latencies instead of all of them) and part 0 adds the
-// others' partial sums in wavefront order.  Part 0 also takes the fall-back grid (read and
-// cleared) first.  Every thread of the workgroup must call this; the result is meaningful in
-// part 0 only.
-template <int NPART, bool kReread>
+// Entries of a node's covering list loaded together (16 float4: 64 registers).
+constexpr int REDUCE_BATCH = 16;
+
+// The staged sum of one node of a reduction block in NPART parts: wavefront w of the workgroup
+// takes entries w, w + NPART, ... of the block's covering list (so a block covered by many chunks
+// waits for fewer memory latencies) and part 0 adds the others' partial sums in wavefront order.
+// Part 0 also takes the fall-back grid (read and cleared) first.  Every thread of the workgroup
+// must call this; the result is meaningful in part 0 only.
+template <int NPART>
 __device__ __forceinline__ float4 staged_node_sum(const float4* __restrict__ staging, const int* s_nc, const int* s_cs,
                                                   float4* __restrict__ fallback, bool fl, long gid, bool inb, int lx,
                                                   int ly, int lz, int ncomp)
 {
     __shared__ float4 s_part[NPART > 1 ? (NPART - 1) * 64 : 1];
-    const int node = threadIdx.x & 63, part = threadIdx.x >> 6;
+    const int node = threadIdx.x & 63, part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (inb) {
-        if (fl && part == 0) {
-            acc = fallback[gid];
-            fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-        sum_covering_slots<kReread>(acc, staging, s_nc, s_cs, lx, ly, lz, ncomp, part, NPART);
+    if (inb && fl && part == 0) {
+        acc = fallback[gid];
+        fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    sum_covering_list<REDUCE_BATCH>(acc, staging, s_nc, s_cs, inb, lx, ly, lz, ncomp, part, NPART);
     if (NPART > 1) {
         if (part > 0) s_part[(part - 1) * 64 + node] = acc;
         __syncthreads();
@@ -479,7 +481,7 @@ __device__ __forceinline__ float4 staged_node_sum(const float4* __restrict__ sta
     return acc;
 }
 
-// One thread per grid node (NPART wavefronts per 4x4x4 node block, see staged_node_sum): the
+// One thread per grid node (NPART wavefronts per reduction block, see staged_node_sum): the
 // sum of the staging slots of the chunks whose tiles cover the node, in a fixed order, plus the
 // fall-back grid (read and cleared for the next step); then the grid update.  Writes the
 // velocity (vout) and, when pout is given, the momentum/mass.
@@ -492,25 +494,26 @@ __global__ void __launch_bounds__(64 * NPART) k_grid_reduce(const float4* __rest
                                                             const int* __restrict__ bnch, const int* __restrict__ cstart,
                                                             const int* __restrict__ cov, float4* __restrict__ fallback,
                                                             int* __restrict__ fbflag, float4* __restrict__ pout,
-                                                            float4* __restrict__ vout, DevScene sc, int NB)
+                                                            float4* __restrict__ vout, DevScene sc, int NB,
+                                                            int NR)
 {
     const int fl = fbflag[blockIdx.x];
     if (!cov[blockIdx.x] && !fl) return;
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
-    const int nb3 = NB * NB * NB;
-    const int inst = blockIdx.x / nb3, blk = blockIdx.x % nb3;
-    const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
+    const int nr3 = NR * NR * NR;
+    const int inst = blockIdx.x / nr3, blk = blockIdx.x % nr3;
+    const int rx = blk / (NR * NR), ry = (blk / NR) % NR, rz = blk % NR;
     const int node = threadIdx.x & 63;
     const int lx = node >> 4, ly = (node >> 2) & 3, lz = node & 3;
-    const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
-    __shared__ int s_nc[3 * 27], s_cs[3 * 27];
-    load_neighbour_buckets(s_nc, s_cs, bnch, cstart, inst, bx, by, bz, NB);
+    const int gx = rx * TB - 1 + lx, gy = ry * TB - 1 + ly, gz = rz * TB - 1 + lz;
+    __shared__ int s_nc[24], s_cs[24];
+    load_covering_buckets(s_nc, s_cs, bnch, cstart, inst, rx, ry, rz, NB);
     __syncthreads();
-    const bool inb = gx < n && gy < n && gz < n;
+    const bool inb = gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n;
     const long gid = (long)inst * nn + (inb ? node_index(n, gx, gy, gz) : 0);
     if (fl && threadIdx.x == 0) fbflag[blockIdx.x] = 0;
-    const float4 acc = staged_node_sum<NPART, false>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 4);
+    const float4 acc = staged_node_sum<NPART>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 4);
     if (!inb || threadIdx.x >= 64) return;
     if (pout) pout[gid] = acc;
     float v[3];
@@ -707,7 +710,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
                                                       const CH* __restrict__ chunks, const int* __restrict__ rows,
                                                       const int* __restrict__ nchunks_p, int maxchunks, const double* __restrict__ dvbar, int T, int f,
                                                       float4* __restrict__ staging, float4* __restrict__ fallback,
-                                                      int* __restrict__ fbflag, int NB, DevScene sc, int P)
+                                                      int* __restrict__ fbflag, int NR, DevScene sc, int P)
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
     const int nchunks = min(*nchunks_p, maxchunks);
@@ -757,7 +760,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
 #pragma unroll
                 for (int a = 0; a < 3; a++) vc[a] = vt[a];
                 if (!it) {
-                    mark_fallback_blocks(fbflag, ch.inst, s, NB);
+                    mark_fallback_blocks(fbflag, ch.inst, s, NR);
                     ScatterGlobal3 g{fallback + (long)ch.inst * nn, n};
                     g2p_adj_scatter(sc, s, vc, Cc, g);
                 }
@@ -793,25 +796,26 @@ __global__ void __launch_bounds__(64 * NPART, 3) k_vbar_reduce(const float4* __r
                                                             const int* __restrict__ bnch, const int* __restrict__ cstart,
                                                             const int* __restrict__ cov, float4* __restrict__ fallback,
                                                             int* __restrict__ fbflag, const float4* __restrict__ grid,
-                                                            float4* __restrict__ pbar, DevScene sc, int NB)
+                                                            float4* __restrict__ pbar, DevScene sc, int NB,
+                                                            int NR)
 {
     const int fl = fbflag[blockIdx.x];
     if (!cov[blockIdx.x] && !fl) return;
     const int n = sc.n_grid;
     const long nn = (long)n * n * n;
-    const int nb3 = NB * NB * NB;
-    const int inst = blockIdx.x / nb3, blk = blockIdx.x % nb3;
-    const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
+    const int nr3 = NR * NR * NR;
+    const int inst = blockIdx.x / nr3, blk = blockIdx.x % nr3;
+    const int rx = blk / (NR * NR), ry = (blk / NR) % NR, rz = blk % NR;
     const int node = threadIdx.x & 63;
     const int lx = node >> 4, ly = (node >> 2) & 3, lz = node & 3;
-    const int gx = bx * TB + lx, gy = by * TB + ly, gz = bz * TB + lz;
-    __shared__ int s_nc[3 * 27], s_cs[3 * 27];
-    load_neighbour_buckets(s_nc, s_cs, bnch, cstart, inst, bx, by, bz, NB);
+    const int gx = rx * TB - 1 + lx, gy = ry * TB - 1 + ly, gz = rz * TB - 1 + lz;
+    __shared__ int s_nc[24], s_cs[24];
+    load_covering_buckets(s_nc, s_cs, bnch, cstart, inst, rx, ry, rz, NB);
     __syncthreads();
-    const bool inb = gx < n && gy < n && gz < n;
+    const bool inb = gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n;
     const long gid = (long)inst * nn + (inb ? node_index(n, gx, gy, gz) : 0);
     if (fl && threadIdx.x == 0) fbflag[blockIdx.x] = 0;
-    const float4 acc = staged_node_sum<NPART, true>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 3);
+    const float4 acc = staged_node_sum<NPART>(staging, s_nc, s_cs, fallback, fl, gid, inb, lx, ly, lz, 3);
     if (!inb || threadIdx.x >= 64) return;
     float v[3];
     NodeRec r;
@@ -1283,7 +1287,7 @@ __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ c
                                                        const int* __restrict__ cstart, const int* __restrict__ gstart,
                                                        int* __restrict__ rrpos, int* __restrict__ rows,
                                                        Chunk* __restrict__ chunks, GChunk* __restrict__ gchunks,
-                                                       int* __restrict__ cov, int NB, int NBK, int rows_per_chunk,
+                                                       int* __restrict__ cov, int NB, int NR, int NBK, int rows_per_chunk,
                                                        int gchunk)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1299,11 +1303,11 @@ __global__ void __launch_bounds__(256) k_bucket_layout(const int* __restrict__ c
     const int nb3 = NB * NB * NB;
     const int inst = bucket / (3 * nb3), sp = (bucket / nb3) % 3, blk = bucket % nb3;
     const int bx = blk / (NB * NB), by = (blk / NB) % NB, bz = blk % NB;
-    if (lane < 27) {
-        // the node blocks this bucket's tiles reach
-        const int cx = bx + lane / 9 - 1, cy = by + (lane / 3) % 3 - 1, cz = bz + lane % 3 - 1;
-        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < NB && cy < NB && cz < NB)
-            cov[(long)inst * nb3 + (cx * NB + cy) * NB + cz] = 1;
+    if (lane < 8) {
+        // the reduction blocks this bucket's tiles reach: b and b + 1 per axis (the tile covers
+        // nodes [4b - 1, 4b + 7))
+        const int cx = bx + ((lane >> 2) & 1), cy = by + ((lane >> 1) & 1), cz = bz + (lane & 1);
+        if (cx < NR && cy < NR && cz < NR) cov[(long)inst * NR * NR * NR + (cx * NR + cy) * NR + cz] = 1;
     }
     int total = nc;
 #pragma unroll
