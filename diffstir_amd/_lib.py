@@ -20,7 +20,8 @@ EXPORTS = (
 )
 
 KERNEL_NAMES = ("k_p2g_tiled", "k_grid_reduce", "k_g2p_tiled", "k_g2p_adj_tiled", "k_vbar_reduce",
-                "k_p2g_adj_tiled", "bucket_sort", "k_p2g_adj_stress", "k_g2p_tiled_factor", "k_p2g_adj_xgrad")
+                "k_p2g_adj_tiled", "bucket_sort", "k_p2g_adj_stress", "k_g2p_tiled_factor", "k_p2g_adj_xgrad",
+                "k_g2p_p2g_tiled")
 
 ERROR_NAMES = {0: "DS_OK", 1: "DS_EINVAL", 2: "DS_EHIP", 3: "DS_ENOMEM", 4: "DS_ESTATE", 5: "DS_ENONFINITE"}
 

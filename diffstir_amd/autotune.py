@@ -30,6 +30,10 @@ them):
 * ``g2p_split`` -- the forward's G2P running each gather chunk as 2 or 4 workgroups: with 2 gather
   particles per thread chosen for the adjoint P2G, the forward's G2P can still run 256-particle
   workgroups (only values up to the chosen particles per thread are tried);
+* ``fused_step`` -- each forward G2P fused with the next step's P2G in one kernel on the
+  scatter's row chunks (row chunks only): it saves a launch and a state round trip per substep
+  where the chip is latency-bound (one instance), and costs where G2P's full gather chunks keep
+  the lanes busier than the rows do (large batches);
 * ``chunk_grid`` -- the launch grid of the chunk kernels: the library's default (16384
   workgroups, of which those past the device-resident chunk count exit at once) or 2048 (a chunk
   count above it loops).  With few chunks (one instance) the smaller grid leaves the two streams
@@ -63,7 +67,7 @@ DEFAULT = {"particles_per_thread": 2, "gather_particles_per_thread": 1, "adjoint
            "scatter_chunks": "rows"}
 
 # options refined after the first stage, in order, with their default (first) values
-LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream", "g2p_split", "chunk_grid")
+LATER_STAGES = ("sort_every", "reduce_parts", "xgrad_stream", "g2p_split", "fused_step", "chunk_grid")
 # a last pass over the options whose best value depends most on the later ones: the rows per
 # wavefront (with the chosen sort interval and reduction split) and then the reduction split
 FINAL_STAGES = ("particles_per_thread", "reduce_parts")
@@ -109,6 +113,9 @@ def stage_candidates(key: str, ckpt_every: int, chosen: Optional[dict] = None) -
     if key == "g2p_split":
         g = (chosen or {}).get("gather_particles_per_thread", 4)
         return [1] + [v for v in (2, 4) if v <= g]
+    if key == "fused_step":
+        # the fused kernel runs on row chunks only
+        return [False, True] if (chosen or {}).get("scatter_chunks", "rows") == "rows" else [False]
     if key == "chunk_grid":
         return [0, 2048]
     raise KeyError(key)
@@ -156,7 +163,7 @@ def warmed_scenes(scenes, engine, warm_steps: int):
     options) reaches after `warm_steps` substeps: liquid and stick positions of that step, the
     stick trajectory from that frame on."""
     import dataclasses
-    engine.configure(**DEFAULT, sort_every=0, reduce_parts=1, xgrad_stream=False, g2p_split=1)
+    engine.configure(**DEFAULT, sort_every=0, reduce_parts=1, xgrad_stream=False, g2p_split=1, fused_step=False)
     engine.reset()
     engine.forward(warm_steps)
     out = []

@@ -49,6 +49,7 @@ struct ds_ctx {
     bool adj_split = false;            // adjoint P2G as two kernels (gather, stress) instead of one
     int rparts = 1;                    // wavefronts per node block of the grid reductions (1, 2, 4)
     int g2p_split = 1;                 // workgroups per gather chunk in G2P (DS_OPT_G2P_SPLIT)
+    bool fuse_step = false;            // G2P fused with the next step's P2G (DS_OPT_FUSED_STEP)
     bool packed = false;               // scatter kernels on the packed (gather) chunks with claim passes
     bool use_graph = false;            // replay whole-rollout forward / adjoint launch sequences as hipGraphs
     bool overlap = true;               // adjoint: recompute segment k-1 on a second stream during segment k's adjoint
@@ -778,13 +779,9 @@ struct Exec {
 };
 inline Exec main_exec(const ds_ctx* c) { return Exec{c->stream, c->d_staging, c->d_fbflag}; }
 
-// P2G of a checkpoint segment's chunks into the staging slots, then the grid reduction:
-// velocity into d_vgrid and, when pout is given, momentum/mass into pout.
-void p2g_reduce_tiled(ds_ctx* c, const Exec& x, const float* in, float* fout, int cp, float4* pout, float* aout)
+// P2G of a checkpoint segment's chunks into the staging slots.
+void p2g_tiled(ds_ctx* c, const Exec& x, const float* in, float* fout, int cp, float* aout)
 {
-    // the scatter's chunk set: rows of distinct stencil bases, or packed gather chunks
-    const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
-    const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
     const int* counts = c->d_counts + 4 * cp;
     {
         KernelTimer kt(c, DS_K_P2G, true);
@@ -799,6 +796,15 @@ void p2g_reduce_tiled(ds_ctx* c, const Exec& x, const float* in, float* fout, in
                                counts + 1, (int)c->maxch, c->d_params, x.staging, c->d_pfb, x.fbflag, c->NR,
                                c->d_j, c->dsc, c->P, c->d_flag);
     }
+}
+
+// The grid reduction of the staging slots that P2G (or the fused G2P + P2G) wrote for table slot
+// cp: velocity into d_vgrid and, when pout is given, momentum/mass into pout.
+void grid_reduce_tiled(ds_ctx* c, const Exec& x, int cp, float4* pout)
+{
+    // the scatter's chunk set: rows of distinct stencil bases, or packed gather chunks
+    const int* bnch = (c->packed ? c->d_bgch : c->d_bnch) + (size_t)cp * c->NBK;
+    const int* cstart = (c->packed ? c->d_gstart : c->d_cstart) + (size_t)cp * c->NBK;
     const int nr3 = c->NR * c->NR * c->NR;
     KernelTimer kt(c, DS_K_GRID_REDUCE, true);
     const int* cov = c->d_cov + (size_t)cp * c->B * nr3;
@@ -827,6 +833,23 @@ void g2p_tiled(ds_ctx* c, const Exec& x, const float* in, float* out, int cp, in
                            c->d_vgrid, c->d_traj, c->T, f, dv, c->d_j, mj, c->dsc, c->P, c->d_flag, split);
 }
 
+// The forward's G2P of step f fused with the P2G of step f + 1 (row chunks of table slot cp, which
+// both steps use): x, v, C of step f + 1 into out, F of step f + 2 into fout, the stick force of
+// step f + 1, the staging slots of step f + 1.
+void g2p_p2g_tiled(ds_ctx* c, const Exec& x, const float* in, const float* fin, float* out, float* fout, int cp,
+                   int f)
+{
+    KernelTimer kt(c, DS_K_G2P_P2G, true);
+    const int* counts = c->d_counts + 4 * cp;
+    DS_PPT_DISPATCH_T(kt, c->ppt_sorted, k_g2p_p2g_tiled, chunk_grid(c, c->maxch), dim3(WG), 0, x.stream, in, fin, out,
+                      fout, c->d_chunks + (size_t)cp * c->maxch, c->d_rows + (size_t)cp * (c->P + 1), counts + 1,
+                      (int)c->maxch, c->d_vgrid, c->d_traj, c->T, f, c->d_dvsum, c->d_params, x.staging, c->d_pfb,
+                      x.fbflag, c->NR, c->dsc, c->P, c->d_flag);
+}
+
+// The fused G2P + P2G applies: the option is on and the scatter runs on row chunks.
+inline bool fused_step(const ds_ctx* c) { return c->fuse_step && !c->packed; }
+
 // J of every liquid particle of the checkpoint state `st` into d_j, for the segment's first P2G
 // (and, given mj, J and its SVD backward factor into mj).
 void liquid_j_tiled(ds_ctx* c, const Exec& x, const float* st, int cp, double* mj = nullptr)
@@ -850,6 +873,8 @@ int forward_tiled(ds_ctx* c, int nsteps)
 {
     const long SF = c->stateFloats();
     const int R = sort_interval(c);
+    // the P2G of the current step was done by the previous step's fused G2P + P2G
+    bool p2g_done = false;
     for (int i = 0; i < nsteps; i++) {
         const int s = c->step;
         const int cp = s / c->K;
@@ -868,8 +893,15 @@ int forward_tiled(ds_ctx* c, int nsteps)
             in = dst;
             liquid_j_tiled(c, main_exec(c), in, slot);
         }
-        p2g_reduce_tiled(c, main_exec(c), in, c->d_state, slot, nullptr, nullptr);
-        g2p_tiled(c, main_exec(c), in, c->d_state, slot, s, true);
+        if (!p2g_done) p2g_tiled(c, main_exec(c), in, c->d_state, slot, nullptr);
+        grid_reduce_tiled(c, main_exec(c), slot, nullptr);
+        // the next step's P2G rides on this step's G2P when it is part of this call and runs under
+        // the same sort (a call ends on a complete state: the last G2P stands alone)
+        p2g_done = fused_step(c) && i + 1 < nsteps && (s + 1) % R != 0;
+        if (p2g_done)
+            g2p_p2g_tiled(c, main_exec(c), in, c->d_state, c->d_state, c->d_state, slot, s);
+        else
+            g2p_tiled(c, main_exec(c), in, c->d_state, slot, s, true);
         c->step++;
     }
     return DS_OK;
@@ -919,7 +951,10 @@ void recompute_segment(ds_ctx* c, const Exec& x, int cp, int S, const SegSet& z)
         // which the forward's permutation of that sort takes to the step's buffer
         const bool resort = k + 1 < L && (k + 1) % R == 0;
         float* next = k + 1 < L ? (resort ? z.tmp : st_of(k + 1)) : nullptr;
-        p2g_reduce_tiled(c, x, st_of(k), next, slot, z.grid + (size_t)k * G, z.aff + (size_t)k * 9 * c->P);
+        // (the recompute keeps G2P and P2G apart: beside the adjoint on the other stream, G2P on
+        // the scatter's sparse rows costs the adjoint more than the saved launch gains)
+        p2g_tiled(c, x, st_of(k), next, slot, z.aff + (size_t)k * 9 * c->P);
+        grid_reduce_tiled(c, x, slot, z.grid + (size_t)k * G);
         if (k + 1 >= L) continue;
         if (!resort) {
             g2p_tiled(c, x, st_of(k), next, slot, s, false, mj_of(k + 1));
@@ -1398,6 +1433,12 @@ int ds_set_option(ds_ctx* c, int32_t option, int32_t value)
         if (value != 0 && value != 1) return fail(c, DS_EINVAL, "DS_OPT_COUNT_FALLBACK takes 0 or 1");
         if (c->dsc.count_fallback != (int)value) drop_graphs(c);
         c->dsc.count_fallback = (int)value;
+        return DS_OK;
+    }
+    if (option == DS_OPT_FUSED_STEP) {
+        if (value != 0 && value != 1) return fail(c, DS_EINVAL, "DS_OPT_FUSED_STEP takes 0 or 1");
+        if (c->fuse_step != (value == 1)) drop_graphs(c);
+        c->fuse_step = value == 1;
         return DS_OK;
     }
     if (option == DS_OPT_ADJOINT_P2G) {

@@ -682,6 +682,187 @@ __global__ void __launch_bounds__(WG) k_liquid_j(const float* __restrict__ st, c
     }
 }
 
+// The forward's G2P of step f fused with the P2G of step f + 1, on the row chunks of one sort (no
+// re-sort between the two steps).  Each lane takes one particle of a row: the gather from the
+// chunk's velocity tile and the particle update of step f (as k_g2p_tiled), then, with the updated
+// particle still in registers, F_tmp, J (FP64 singular values), the stress and the scatter of step
+// f + 1 into the wavefront's tile (as k_p2g_tiled).  One launch instead of two per substep, and the
+// particle state is not written by one kernel and read back by the next; G2P then runs on the
+// scatter's rows (one particle per occupied cell) instead of full gather chunks.
+//   in:   state of step f (x is read)          fin:  F of step f + 1 (written by the previous P2G)
+//   out:  x, v, C of step f + 1                fout: F of step f + 2
+// In the forward in, fin, out and fout are one buffer (each thread reads its particle before it
+// writes it), so they are not restrict-qualified.  The claim table holds only the bases a tile
+// takes (node keys up to 365) and the velocity tile has dense planes, so four workgroups fit the
+// LDS of a CU.
+constexpr int OWN_KEYS = 368;   // node_key of a base in [0, 5]^3 is at most (5 * 8 + 5) * 8 + 5 = 365
+using PlanesD = Planes<TS * TS, TS>;
+
+template <int PPT>
+__global__ void __launch_bounds__(WG, 4) k_g2p_p2g_tiled(const float* in, const float* fin, float* out, float* fout,
+                                                         const Chunk* __restrict__ chunks,
+                                                         const int* __restrict__ rows, const int* __restrict__ nchunks_p,
+                                                         int maxchunks, const float4* __restrict__ vgrid,
+                                                         const float* __restrict__ traj, int T, int f,
+                                                         double* __restrict__ dvsum, const double* __restrict__ params,
+                                                         float4* __restrict__ staging, float4* __restrict__ fallback,
+                                                         int* __restrict__ fbflag, int NR, DevScene sc, int P,
+                                                         unsigned* __restrict__ flag)
+{
+    // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
+    const int nchunks = min(*nchunks_p, maxchunks);
+    __shared__ float4 wt[WAVES * TNODES];
+    __shared__ unsigned char own[WAVES * OWN_KEYS];
+    __shared__ float tv[3 * PlanesD::size];
+    __shared__ double red[WAVES][3];
+    const int n = sc.n_grid;
+    const long nn = (long)n * n * n;
+    const long LP = P;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    unsigned char* ow = own + wave * OWN_KEYS;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const Chunk ch = chunks[c];
+        const int x0 = ch.ox - 1, y0 = ch.oy - 1, z0 = ch.oz - 1;
+        const bool liq = ch.species == LIQUID;
+        const float4* vi = vgrid + (long)ch.inst * nn;
+        // the first row's particles are loaded before the tiles are set up, and each next row's
+        // while the current one scatters
+        int p = 0;
+        bool valid = scatter_slot(ch, rows, 0, wave, lane, p);
+        float xs[3], Fs[9];
+        auto load_xf = [&](int pp) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) xs[a] = in[(SX + a) * LP + pp];
+#pragma unroll
+            for (int a = 0; a < 9; a++)
+                Fs[a] = liq ? fin[(SF + a) * LP + pp] : ((a == 0 || a == 4 || a == 8) ? 1.0f : 0.0f);
+        };
+        if (valid) load_xf(p);
+        for (int e = threadIdx.x; e < TNODES; e += WG) {
+            const int lx = e / (TS * TS), ly = (e / TS) % TS, lz = e % TS;
+            const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n) v = vi[node_index(n, gx, gy, gz)];
+            const int id = PlanesD::id(lx, ly, lz);
+            tv[id] = v.x;
+            tv[PlanesD::size + id] = v.y;
+            tv[2 * PlanesD::size + id] = v.z;
+        }
+        zero_lds(reinterpret_cast<float*>(wt), WAVES * 4 * TNODES);
+        const InstParams ip = load_params(sc, params, ch.inst);
+        WaveScatter4 acc{wt + wave * TNODES, x0, y0, z0};
+        const float* tr = traj + ((long)ch.inst * T + f) * 3;
+        double dv[3] = {0.0, 0.0, 0.0};
+        __syncthreads();   // velocity tile loaded, wave tiles cleared
+#pragma unroll 1
+        for (int r = 0; r < PPT; r++) {
+            Stencil s;
+            P2GPayload pl;
+            bool it = false;
+            if (valid) {
+                // G2P of step f
+                ParticleIn q;
+                {
+                    Stencil sg;
+                    if (!stencil(sc, xs[0], xs[1], xs[2], sg)) atomicOr(flag, 1u);
+                    float nv[3];
+                    if (stencil_in_tile(sg, x0, y0, z0, sc.tile_win)) {
+                        TileVT<PlanesD> g{tv, x0, y0, z0};
+                        g2p_gather(sc, sg, g, nv, q.C);
+                    } else {
+                        GatherVout g{vi, n};
+                        g2p_gather(sc, sg, g, nv, q.C);
+                    }
+                    if (liq) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            q.v[a] = nv[a];
+                            q.x[a] = xs[a] + sc.dt * nv[a];
+                        }
+                    } else if (ch.species == STICK) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            const float vk = (tr[3 + a] - tr[a]) / sc.dt;
+                            dv[a] += (double)(vk - nv[a]);
+                            q.v[a] = vk;
+                            q.x[a] = xs[a] + vk * sc.dt;
+                        }
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            q.v[a] = 0.0f;
+                            q.x[a] = xs[a];
+                        }
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        out[(SX + a) * LP + p] = q.x[a];
+                        out[(SV + a) * LP + p] = q.v[a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 9; a++) out[(SC + a) * LP + p] = q.C[a];
+#pragma unroll
+                    for (int a = 0; a < 9; a++) q.F[a] = Fs[a];
+                }
+                // P2G of step f + 1
+                if (!stencil(sc, q.x[0], q.x[1], q.x[2], s)) atomicOr(flag, 1u);
+                double J = 1.0;
+                if (liq) {
+                    float Ft[9];
+                    f_tmp(sc, q.C, q.F, Ft);
+                    J = liquid_j(Ft);
+                }
+                float Fn[9];
+                p2g_prepare(sc, ch.species, ip, q, Fn, pl, &J);
+                if (liq) {
+#pragma unroll
+                    for (int a = 0; a < 9; a++) fout[(SF + a) * LP + p] = Fn[a];
+                }
+                it = stencil_in_tile(s, x0, y0, z0, sc.tile_win);
+                if (!it) {
+                    mark_fallback_blocks(fbflag, ch.inst, s, NR);
+                    if (sc.count_fallback) atomicAdd(reinterpret_cast<unsigned long long*>(flag + 2), 1ull);
+                    ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
+                    p2g_scatter(sc, s, pl, g);
+                }
+            }
+            int pn = 0;
+            const bool vn = r + 1 < PPT && scatter_slot(ch, rows, r + 1, wave, lane, pn);
+            if (vn) load_xf(pn);
+            bool pending = valid && it;
+            const int key = pending ? node_key(s, x0, y0, z0) : 0;
+            while (__ballot(pending)) {
+                lds_order();
+                if (pending) ow[key] = (unsigned char)lane;
+                lds_order();
+                if (pending && ow[key] == lane) {
+                    p2g_scatter(sc, opaque(s), pl, acc);
+                    pending = false;
+                }
+            }
+            valid = vn;
+            p = pn;
+        }
+        if (ch.species == STICK && dvsum) {
+            const double s0 = wave_sum(dv[0]), s1 = wave_sum(dv[1]), s2 = wave_sum(dv[2]);
+            if (lane == 0) {
+                red[wave][0] = s0;
+                red[wave][1] = s1;
+                red[wave][2] = s2;
+            }
+            __syncthreads();
+            if (threadIdx.x < 3) {
+                double acc2 = 0.0;
+                for (int k = 0; k < WAVES; k++) acc2 += red[k][threadIdx.x];
+                atomicAdd(dvsum + ((long)ch.inst * T + f + 1) * 3 + threadIdx.x, exact_grain(acc2));
+            }
+        }
+        __syncthreads();
+        store_staging4(wt, staging + (long)c * TNODES);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------- adjoint
 
 // Velocity tile (layout L) from the momentum/mass grid: the grid update at every node of the tile.

@@ -47,7 +47,7 @@ class Engine:
     # execution options of the tiled kernels (ds_set_option ids)
     OPT_KERNELS, OPT_PPT, OPT_TIMING, OPT_GATHER_PPT, OPT_ADJ_P2G, OPT_SCATTER, OPT_GRAPH = 1, 2, 3, 4, 5, 6, 7
     OPT_OVERLAP, OPT_CHUNK_GRID, OPT_SORT_EVERY, OPT_REDUCE_PARTS, OPT_XGRAD_STREAM = 8, 9, 10, 11, 12
-    OPT_G2P_SPLIT, OPT_TILE_WINDOW, OPT_COUNT_FALLBACK = 13, 14, 15
+    OPT_G2P_SPLIT, OPT_TILE_WINDOW, OPT_COUNT_FALLBACK, OPT_FUSED_STEP = 13, 14, 15, 16
     ADJ_P2G = {"fused": 0, "split": 1}
     SCATTER = {"rows": 0, "packed": 1}
 
@@ -91,6 +91,7 @@ class Engine:
         self.reduce_parts = 1
         self.xgrad_stream = False
         self.g2p_split = 1
+        self.fused_step = False
         self.chunk_grid = 0
         self.set_graph(graph)
         for b, s in enumerate(scenes):
@@ -116,7 +117,8 @@ class Engine:
                   adjoint_p2g: Optional[str] = None, scatter_chunks: Optional[str] = None,
                   overlap: Optional[bool] = None, chunk_grid: Optional[int] = None,
                   sort_every: Optional[int] = None, reduce_parts: Optional[int] = None,
-                  xgrad_stream: Optional[bool] = None, g2p_split: Optional[int] = None):
+                  xgrad_stream: Optional[bool] = None, g2p_split: Optional[int] = None,
+                  fused_step: Optional[bool] = None):
         """Execution options of the tiled kernels (they change how the substep runs, never what it
         computes); particle-per-thread changes take effect at the next reset.  `overlap`: the
         adjoint recomputes the next checkpoint segment on a second stream while the current one's
@@ -126,7 +128,9 @@ class Engine:
         wavefronts per node block of the grid reductions (1, 2 or 4).  `xgrad_stream`: the adjoint
         P2G's position gradient in its own kernel on a third stream, beside the adjoint G2P and
         grid reduction of the same step.  `g2p_split`: workgroups per gather chunk in the forward's G2P
-        (1, 2 or 4, at most the gather particles per thread)."""
+        (1, 2 or 4, at most the gather particles per thread).  `fused_step`: each forward G2P runs
+        together with the next step's P2G in one kernel on the scatter's row chunks, where no sort
+        falls between them and both steps are in one forward() call."""
         if particles_per_thread is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_PPT, int(particles_per_thread)))
             self.particles_per_thread = int(particles_per_thread)
@@ -161,6 +165,9 @@ class Engine:
         if g2p_split is not None:
             self._check(self.lib.ds_set_option(self.ctx, self.OPT_G2P_SPLIT, int(g2p_split)))
             self.g2p_split = int(g2p_split)
+        if fused_step is not None:
+            self._check(self.lib.ds_set_option(self.ctx, self.OPT_FUSED_STEP, 1 if fused_step else 0))
+            self.fused_step = bool(fused_step)
 
     def set_diagnostics(self, tile_window: Optional[int] = None, count_fallback: Optional[bool] = None):
         """Diagnostics of the tiled kernels: `tile_window` (0..5, default 5) narrows the window of
@@ -196,7 +203,7 @@ class Engine:
                 "gather_particles_per_thread": self.gather_particles_per_thread, "adjoint_p2g": self.adjoint_p2g,
                 "scatter_chunks": self.scatter_chunks, "overlap": self.overlap, "sort_every": self.sort_every,
                 "reduce_parts": self.reduce_parts, "xgrad_stream": self.xgrad_stream, "g2p_split": self.g2p_split,
-                "chunk_grid": self.chunk_grid}
+                "fused_step": self.fused_step, "chunk_grid": self.chunk_grid}
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):

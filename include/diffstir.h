@@ -79,6 +79,11 @@ enum {
                                           grid.  Same results, to the FP32 order of the fall-back atomics */
     DS_OPT_COUNT_FALLBACK = 15,        /* diagnostic: 1 counts the stencils P2G sends to the fall-back grid
                                           (ds_fallback_count); 0 (default): no counting */
+    DS_OPT_FUSED_STEP = 16,            /* tiled kernels with row chunks: 1 runs each forward G2P together
+                                          with the next step's P2G in one kernel (k_g2p_p2g_tiled) wherever
+                                          no sort falls between them and both steps are in the same
+                                          ds_forward call (the adjoint's recompute keeps them apart);
+                                          0 (default): separate G2P and P2G launches */
 };
 enum { DS_ADJ_P2G_FUSED = 0, DS_ADJ_P2G_SPLIT = 1 };
 /* Scatter kernels (P2G, adjoint G2P): chunks of rows whose lanes have distinct stencil bases, or
@@ -97,7 +102,8 @@ enum {
     DS_K_P2G_ADJ_STRESS = 7, /* k_p2g_adj_stress: adjoint of the stress and of the FP64 SVD */
     DS_K_G2P_FACTOR = 8,   /* k_g2p_tiled of the recompute, with the full FP64 SVD */
     DS_K_P2G_ADJ_XGRAD = 9, /* k_p2g_adj_xgrad: the adjoint P2G's position gradient */
-    DS_NKERNELS = 10
+    DS_K_G2P_P2G = 10,     /* k_g2p_p2g_tiled: G2P fused with the next step's P2G */
+    DS_NKERNELS = 11
 };
 enum { DS_KERNELS_TILED = 0, DS_KERNELS_PER_PARTICLE = 1 };
 

@@ -51,14 +51,15 @@ def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
         if kernel_opts != want:
             return 2.0 + len(calls)
         # among the later stages' values: sorting once per interval wins, 2 reduction parts win,
-        # the position-gradient stream wins
+        # the position-gradient stream wins, the fused G2P + P2G wins
         return (1.0 + (0.5 if cand["sort_every"] else 0.0) + (0.0 if cand["reduce_parts"] == 2 else 0.25)
-                + (0.0 if cand["xgrad_stream"] else 0.125) + 0.0625 * (cand["g2p_split"] - 1))
+                + (0.0 if cand["xgrad_stream"] else 0.125) + 0.0625 * (cand["g2p_split"] - 1)
+                + (0.0 if cand["fused_step"] else 0.03125))
 
     eng = _FakeEngine()
     f = tmp_path / "tune.json"
     res = A.tune(scenes, ckpt_every=10, engine=eng, timer=timer, cache_file=str(f), warm_steps=0)
-    want2 = dict(want, sort_every=0, reduce_parts=2, xgrad_stream=True, g2p_split=1, chunk_grid=0)
+    want2 = dict(want, sort_every=0, reduce_parts=2, xgrad_stream=True, g2p_split=1, fused_step=True, chunk_grid=0)
     assert res["options"] == want2
     assert eng.configured[-1] == want2
     nc = len(A.candidates())
@@ -66,21 +67,25 @@ def test_tune_uses_the_timer_configures_the_winner_and_caches(tmp_path):
     assert n2 == 2
     T = scenes[0].max_steps
     # later stages: 2 sort intervals, 2 reduction splits, the stream, G2P split 2 (gppt 2 chosen),
-    # the smaller launch grid; then the last pass: rows per wavefront 1 and 2, reduction splits 1 and 4
-    assert len(calls) == nc + n2 + 5 + 4 and all(s == min(T - 1, 20) for _, s in calls)   # 2 checkpoint segments at most
+    # the fused G2P + P2G, the smaller launch grid; then the last pass: rows per wavefront 1 and 2,
+    # reduction splits 1 and 4
+    assert len(calls) == nc + n2 + 6 + 4 and all(s == min(T - 1, 20) for _, s in calls)   # 2 checkpoint segments at most
     # first stage: every kernel candidate with the later options at their defaults
     assert [c for c, _ in calls[:nc]] == [dict(c, sort_every=0, reduce_parts=1, xgrad_stream=False, g2p_split=1,
-                                               chunk_grid=0) for c in A.candidates()]
+                                               fused_step=False, chunk_grid=0) for c in A.candidates()]
     # then the sort intervals, the reduction parts and the position-gradient stream, each with the
     # options chosen so far
     assert [c for c, _ in calls[nc:nc + n2]] == [dict(want, sort_every=R, reduce_parts=1, xgrad_stream=False,
-                                                      g2p_split=1, chunk_grid=0) for R in A.sort_candidates(10)[1:]]
+                                                      g2p_split=1, fused_step=False, chunk_grid=0)
+                                                 for R in A.sort_candidates(10)[1:]]
     assert [c for c, _ in calls[nc + n2:nc + n2 + 2]] == [dict(want, sort_every=0, reduce_parts=p, xgrad_stream=False,
-                                                               g2p_split=1, chunk_grid=0) for p in (2, 4)]
-    assert calls[nc + n2 + 2][0] == want2
-    assert calls[nc + n2 + 3][0] == dict(want2, g2p_split=2)
-    assert calls[nc + n2 + 4][0] == dict(want2, chunk_grid=2048)
-    assert [c for c, _ in calls[nc + n2 + 5:]] == [dict(want2, particles_per_thread=1), dict(want2, particles_per_thread=2),
+                                                               g2p_split=1, fused_step=False, chunk_grid=0)
+                                                          for p in (2, 4)]
+    assert calls[nc + n2 + 2][0] == dict(want2, fused_step=False)
+    assert calls[nc + n2 + 3][0] == dict(want2, g2p_split=2, fused_step=False)
+    assert calls[nc + n2 + 4][0] == want2
+    assert calls[nc + n2 + 5][0] == dict(want2, chunk_grid=2048)
+    assert [c for c, _ in calls[nc + n2 + 6:]] == [dict(want2, particles_per_thread=1), dict(want2, particles_per_thread=2),
                                                   dict(want2, reduce_parts=1), dict(want2, reduce_parts=4)]
     # cached in memory and on disk by signature
     again = A.tune(scenes, ckpt_every=10, engine=eng, timer=lambda *a: pytest.fail("re-timed"))
@@ -125,7 +130,9 @@ def test_sort_candidates_divide_the_interval():
 
 def test_later_stage_defaults_are_the_library_defaults():
     assert A.later_defaults(50) == {"sort_every": 0, "reduce_parts": 1, "xgrad_stream": False, "g2p_split": 1,
-                                    "chunk_grid": 0}
+                                    "fused_step": False, "chunk_grid": 0}
+    assert A.stage_candidates("fused_step", 50, {"scatter_chunks": "rows"}) == [False, True]
+    assert A.stage_candidates("fused_step", 50, {"scatter_chunks": "packed"}) == [False]
     assert A.stage_candidates("g2p_split", 50, {"gather_particles_per_thread": 1}) == [1]
     assert A.stage_candidates("g2p_split", 50, {"gather_particles_per_thread": 2}) == [1, 2]
     assert A.stage_candidates("reduce_parts", 50) == [1, 2, 4]

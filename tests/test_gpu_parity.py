@@ -275,13 +275,16 @@ def test_autotune_picks_a_candidate_on_the_device():
     assert opts.pop("xgrad_stream") in (False, True)
     assert opts.pop("g2p_split") <= opts["gather_particles_per_thread"]
     assert opts.pop("chunk_grid") in (0, 2048)
+    assert opts.pop("fused_step") in ((False, True) if opts["scatter_chunks"] == "rows" else (False,))
     assert opts in A.candidates()
     assert res["warm_steps"] == min((sc.max_steps - 1) // 2, sc.max_steps - 1 - 12)
     n_split = len(A.stage_candidates("g2p_split", 6, opts)) - 1
+    n_fused = len(A.stage_candidates("fused_step", 6, opts)) - 1
     # the last pass: the other two rows-per-wavefront values (row chunks only), the other two
     # reduction splits
     n_last = (2 if opts["scatter_chunks"] == "rows" else 0) + 2
-    assert len(res["timings_ms"]) == len(A.candidates()) + len(A.sort_candidates(6)) - 1 + 2 + 1 + n_split + 1 + n_last
+    assert len(res["timings_ms"]) == (len(A.candidates()) + len(A.sort_candidates(6)) - 1 + 2 + 1 + n_split + n_fused
+                                      + 1 + n_last)
     assert all(np.isfinite(t["ms"]) and t["ms"] > 0 for t in res["timings_ms"])
     A.clear_cache()
 
@@ -508,15 +511,67 @@ def test_g2p_split_gives_the_same_result():
         np.testing.assert_allclose(d, da, rtol=1e-10, atol=1e-10 * np.abs(da).max(), err_msg=str(key))
 
 
+def test_fused_g2p_p2g_gives_the_same_result():
+    """The forward's G2P fused with the next step's P2G (one kernel on the row chunks wherever no
+    sort falls between the two steps) gives the separate kernels' particle state bit for bit at the
+    end of a rollout and, with a loss gradient that does not depend on the forces, the same
+    gradient bit for bit (the recompute of the adjoint replays the same states): the per-particle
+    arithmetic, the rows and the per-node sum order are the same.  The forces
+    agree to the rounding of the stick-force partials, which the two forms group into different
+    workgroups.  Cases: several forward calls (each ends on a separate G2P), sorts inside the
+    checkpoint interval, 1 and 4 rows per wavefront, the overlapped recompute, graph replay; and two
+    fused runs have the same bits."""
+    sc = S.stirring_scene(32, 150, fill_radius=0.25, fill_height=0.2, stick_radius=0.05, path_rx=0.15, path_rz=0.15,
+                          omega=2 * np.pi * 3, seed=2)
+    res = {}
+    for fused, R, ppt, graph, overlap in ((False, 0, 2, False, True), (True, 0, 2, False, True),
+                                          (True, 0, 2, True, True), (False, 30, 1, False, False),
+                                          (True, 30, 1, False, False), (True, 30, 1, True, True),
+                                          (False, 12, 4, False, True), (True, 12, 4, False, True)):
+        print(f"[gpu] fused G2P + P2G {fused}, sort every {R or 60}, rows per wavefront {ppt}, graph {graph}, "
+              f"overlap {overlap}", flush=True)
+        with Engine([sc], ckpt_every=60, graph=graph, overlap=overlap) as e:
+            e.configure(sort_every=R, particles_per_thread=ppt, fused_step=fused)
+            assert e.options()["fused_step"] == fused
+            e.reset()
+            if graph:
+                e.forward()
+            else:
+                e.forward(7)
+                e.forward(50)
+                e.forward()
+            F = e.forces()[0]
+            st = e.read_liquid(0)
+            d = e.backward(np.ones_like(F)[None])[0]
+            res[(fused, R, ppt, graph, overlap)] = (F, st, d)
+    for (fused, R, ppt, graph, overlap), (F, st, d) in res.items():
+        if not fused:
+            continue
+        key = (fused, R, ppt, graph, overlap)
+        Fa, sa, da = next(v for k, v in res.items() if not k[0] and k[1] == R and k[2] == ppt)
+        for k in ("x", "v", "C", "F"):
+            np.testing.assert_array_equal(st[k], sa[k], err_msg=str(key))
+        np.testing.assert_array_equal(d, da, err_msg=str(key))
+        np.testing.assert_allclose(F, Fa, rtol=1e-10, atol=1e-10 * np.abs(Fa).max(), err_msg=str(key))
+    # repeat runs of the fused form: the same bits, launched and replayed
+    Fa, _, da = res[(True, 0, 2, False, True)]
+    Fb, _, db = res[(True, 0, 2, True, True)]
+    np.testing.assert_array_equal(Fb, Fa)
+    np.testing.assert_array_equal(db, da)
+
+
 @pytest.mark.parametrize("opts", [
     dict(sort_every=5, reduce_parts=4, xgrad_stream=True, g2p_split=2, gather_particles_per_thread=2),
+    dict(sort_every=5, reduce_parts=2, fused_step=True, particles_per_thread=1),
+    dict(fused_step=True, particles_per_thread=4, xgrad_stream=True, adjoint_p2g="split"),
     dict(sort_every=7, reduce_parts=2, xgrad_stream=True, adjoint_p2g="split", scatter_chunks="packed"),
     dict(reduce_parts=2, xgrad_stream=False, g2p_split=4, gather_particles_per_thread=4, particles_per_thread=1),
 ])
 def test_later_stage_options_give_the_oracle_result(opts):
     """Combinations of the autotuner's later-stage options (sort interval, reduction parts, the
-    position-gradient stream, G2P split) with the kernel options give the oracle's forces and
-    gradient on the golden n_grid 32 scene, launched and replayed as a hipGraph."""
+    position-gradient stream, G2P split, G2P fused with the next step's P2G) with the kernel options
+    give the oracle's forces and gradient on the golden n_grid 32 scene, launched and replayed as a
+    hipGraph."""
     sc, z = gold_scene("plain_n32_20")
     for graph in (False, True):
         print(f"[gpu] options {opts}, graph {graph}", flush=True)

@@ -117,7 +117,8 @@ def test_engine_option_numbers_match_the_header():
             "OPT_OVERLAP": "DS_OPT_OVERLAP", "OPT_CHUNK_GRID": "DS_OPT_CHUNK_GRID",
             "OPT_SORT_EVERY": "DS_OPT_SORT_EVERY", "OPT_REDUCE_PARTS": "DS_OPT_REDUCE_PARTS",
             "OPT_XGRAD_STREAM": "DS_OPT_XGRAD_STREAM", "OPT_G2P_SPLIT": "DS_OPT_G2P_SPLIT",
-            "OPT_TILE_WINDOW": "DS_OPT_TILE_WINDOW", "OPT_COUNT_FALLBACK": "DS_OPT_COUNT_FALLBACK"}
+            "OPT_TILE_WINDOW": "DS_OPT_TILE_WINDOW", "OPT_COUNT_FALLBACK": "DS_OPT_COUNT_FALLBACK",
+            "OPT_FUSED_STEP": "DS_OPT_FUSED_STEP"}
     for attr, name in want.items():
         assert getattr(Engine, attr) == enum[name], name
     assert len([k for k in enum if k.startswith("DS_OPT_")]) == len(want)
