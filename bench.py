@@ -54,6 +54,7 @@ KERNEL_BYTES_PER_PARTICLE = {
     "k_g2p_tiled_factor": 12.0 + 36.0 + 60.0 + 8.0 + 80.0,  # ... and write J, M (the recompute)
     "k_g2p_adj_tiled": 12.0 + 60.0,                       # read x and the adjoints of x, v, C
     "k_p2g_adj_stress": 72.0 + 72.0 + 36.0 + 80.0 + 96.0,  # read C, F, gathered, adj F, J/M; write adj
+    "k_g2p_p2g_tiled": 12.0 + 36.0 + 60.0 + 36.0,         # read x, F; write x, v, C, F (G2P + next P2G)
 }
 # the adjoint P2G gathers: read x, v, A and the adjoints of x, v, C; then either the stress half
 # too (fused: read C, F, adj F, J/M, write the adjoint) or the 18 gathered floats (split)
