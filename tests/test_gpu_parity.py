@@ -28,11 +28,20 @@ def o64():
     return Oracle("f64")
 
 
-KERNELS = ["tiled", "per_particle"]
+# kernel families; "tiled_fused": the tiled kernels with the forward's G2P fused into the next
+# step's P2G (DS_OPT_FUSED_STEP)
+KERNELS = ["tiled", "per_particle", "tiled_fused"]
+
+
+def make_engine(scenes, kernels="tiled", **kw):
+    e = Engine(scenes, kernels="tiled" if kernels == "tiled_fused" else kernels, **kw)
+    if kernels == "tiled_fused":
+        e.configure(fused_step=True)
+    return e
 
 
 def gpu_rollout_grad(sc, nsteps, target, ckpt=10, kernels="tiled"):
-    with Engine([sc], ckpt_every=ckpt, max_steps=nsteps + 1, kernels=kernels) as e:
+    with make_engine([sc], kernels, ckpt_every=ckpt, max_steps=nsteps + 1) as e:
         e.reset()
         e.forward(nsteps)
         F = e.forces()[0]
@@ -89,7 +98,7 @@ def test_batch_matches_single_instances(kernels):
     scenes = [S.tiny_scene(16, 25, seed=s, fill_height=h, params=S.hb_params_draw(np.random.default_rng(s)))
               for s, h in [(0, 0.15), (1, 0.25), (2, 0.2)]]
     print(f"[gpu] batch of 3 vs singles ({kernels})", [s.n_liquid for s in scenes], flush=True)
-    with Engine(scenes, ckpt_every=5, kernels=kernels) as e:
+    with make_engine(scenes, kernels, ckpt_every=5) as e:
         e.reset()
         e.forward()
         Fb = e.forces()
@@ -123,7 +132,7 @@ def test_stick_alone_gives_centripetal_force(kernels):
     cfg = dataclasses.replace(sc.cfg, wall_r=10.0)
     sc = dataclasses.replace(sc, cfg=cfg, x_liquid=np.zeros((0, 3), np.float32))
     print(f"[gpu] stick alone: {sc.n_stick} particles", flush=True)
-    with Engine([sc], ckpt_every=10, kernels=kernels) as e:
+    with make_engine([sc], kernels, ckpt_every=10) as e:
         e.reset()
         e.forward()
         F = e.forces()[0]
@@ -290,14 +299,14 @@ def test_autotune_picks_a_candidate_on_the_device():
 
 
 def test_tiled_matches_per_particle_kernels_over_a_longer_rollout():
-    """The two kernel families on the 18.5k-particle n_grid 32 scene for 150 substeps with two
-    re-sorts per checkpoint interval of 60: state, forces and gradient agree to FP32
-    accumulation-order noise."""
+    """The two kernel families (the tiled one with and without the fused forward step) on the
+    18.5k-particle n_grid 32 scene for 150 substeps with two re-sorts per checkpoint interval of 60:
+    state, forces and gradient agree to FP32 accumulation-order noise."""
     sc = S.workload("plain_n32")[0].truncated(151)
     print(f"[gpu] tiled vs per-particle: {sc.n_liquid} particles, 150 steps", flush=True)
     out = {}
     for kernels in KERNELS:
-        with Engine([sc], ckpt_every=60, kernels=kernels) as e:
+        with make_engine([sc], kernels, ckpt_every=60) as e:
             e.reset()
             e.forward()
             F = e.forces()[0]
@@ -306,12 +315,15 @@ def test_tiled_matches_per_particle_kernels_over_a_longer_rollout():
             _, g = compute_loss(F, 0.5 * F, 1)
             d = e.backward(g[None])[0]
         out[kernels] = (F, st, stick, d)
-    (Ft, st_t, sk_t, dt_), (Fp, st_p, sk_p, dp) = out["tiled"], out["per_particle"]
-    assert force_err(Ft, Fp) <= 1e-5
-    for k in ("x", "v", "C", "F"):
-        np.testing.assert_allclose(st_t[k], st_p[k], rtol=1e-4, atol=1e-5 * np.abs(st_p[k]).max())
-    np.testing.assert_allclose(sk_t["x"], sk_p["x"], atol=1e-6)
-    assert grad_err(dt_, dp) <= 1e-4
+    Fp, st_p, sk_p, dp = out["per_particle"]
+    for kernels in ("tiled", "tiled_fused"):
+        Ft, st_t, sk_t, dt_ = out[kernels]
+        assert force_err(Ft, Fp) <= 1e-5, kernels
+        for k in ("x", "v", "C", "F"):
+            np.testing.assert_allclose(st_t[k], st_p[k], rtol=1e-4, atol=1e-5 * np.abs(st_p[k]).max(),
+                                       err_msg=kernels)
+        np.testing.assert_allclose(sk_t["x"], sk_p["x"], atol=1e-6, err_msg=kernels)
+        assert grad_err(dt_, dp) <= 1e-4, kernels
 
 
 # ------------------------------------------------------------------------- FP64 SVD

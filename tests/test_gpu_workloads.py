@@ -162,15 +162,19 @@ def test_single_n64_full_rollout_gradient_matches_central_differences():
 
 # ----------------------------------------------------------------------------- fall-back scatter
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("window", [0, 1, 3])
-def test_fallback_scatter_matches_the_fixture(window):
+def test_fallback_scatter_matches_the_fixture(window, fused):
     """The tiled kernels with a narrowed tile window: most stencils go through the global
-    fall-back grid (P2G, the adjoint G2P, and the gathers read the global grids).  The full
-    18.5k-particle n_grid-32 scene over 20 substeps against its committed oracle fixture."""
+    fall-back grid (P2G, the adjoint G2P, and the gathers read the global grids); with `fused`,
+    the forward's G2P runs inside the next step's P2G kernel and its out-of-window gathers and
+    scatters take the same global paths.  The full 18.5k-particle n_grid-32 scene over 20
+    substeps against its committed oracle fixture."""
     sc, z = gold_scene("plain_n32_20")
     n = int(z["nsteps"])
-    print(f"[gpu] fall-back scatter, tile window {window}", flush=True)
+    print(f"[gpu] fall-back scatter, tile window {window}, fused forward step {fused}", flush=True)
     with Engine([sc], ckpt_every=7, max_steps=n + 1) as e:
+        e.configure(fused_step=fused)
         e.set_diagnostics(tile_window=window, count_fallback=True)
         e.reset()
         e.forward()
