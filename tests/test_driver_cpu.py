@@ -153,3 +153,67 @@ def test_reference_api_constants_and_host_fields(tmp_path):
     with pytest.raises(IndexError):
         R.loss[0]
     assert R.ti.max(abs(-3.0), 1e-11) == 3.0 and R.ti.i32 is int and R.ti.f32 is float
+
+
+class _ModelEngine:
+    """Stands in for Engine in identify_lm: forces from a smooth closed form of the parameters,
+    with a DS_ENONFINITE failure on the forward calls listed in `fail_on`."""
+    T = 40
+    calls = 0
+    fail_on = ()
+
+    def __init__(self, scenes, ckpt_every=50):
+        self.n = len(scenes)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        pass
+
+    @classmethod
+    def profile(cls, p):
+        t = np.arange(cls.T) / cls.T
+        F = np.zeros((cls.T, 3))
+        F[:, 0] = p[0] * np.exp(-p[2] * t) + p[1] * t ** 2
+        F[:, 2] = p[0] * p[1] * np.sin(5 * t) + p[2]
+        return F
+
+    def set_params(self, P):
+        assert P.shape == (self.n, 4)
+        self.P = np.array(P)
+
+    def reset(self):
+        pass
+
+    def forward(self):
+        from diffstir_amd._lib import DiffStirError
+        type(self).calls += 1
+        if type(self).calls in self.fail_on:
+            raise DiffStirError(5, "a particle left the grid's interior")
+
+    def forces(self):
+        return np.stack([self.profile(p) for p in self.P])
+
+
+@pytest.mark.parametrize("fail_on", [(), (2, 3, 7)])
+def test_levenberg_marquardt_bookkeeping(monkeypatch, fail_on):
+    """identify_lm on a closed-form force model: from starts 0.3x-3x off, the best loss never
+    rises, batches that fail with DS_ENONFINITE count as rejected steps, and the ground truth is
+    reached; a failure before any restart has a point raises."""
+    from diffstir_amd import identify as I
+    from diffstir_amd._lib import DiffStirError
+    monkeypatch.setattr(_ModelEngine, "calls", 0)
+    monkeypatch.setattr(_ModelEngine, "fail_on", fail_on)
+    monkeypatch.setattr(I, "Engine", _ModelEngine)
+    truth = np.array([2.0, 15.0, 1.4, 1000.0])
+    inits = np.stack([truth * np.r_[0.3, 3.0, 0.75, 1.0], truth * np.r_[1.3, 0.8, 1.2, 1.0]])
+    res = I.identify_lm(None, _ModelEngine.profile(truth), inits, iterations=25)
+    L = res["losses"]
+    assert L.shape == (26, 2) and res["params"].shape == (26, 2, 4)
+    assert np.all(np.diff(L, axis=0) <= 0)
+    np.testing.assert_allclose(res["final"], np.stack([truth, truth]), rtol=1e-8)
+    monkeypatch.setattr(_ModelEngine, "calls", 0)
+    monkeypatch.setattr(_ModelEngine, "fail_on", (1,))
+    with pytest.raises(DiffStirError):
+        I.identify_lm(None, _ModelEngine.profile(truth), inits, iterations=2)
