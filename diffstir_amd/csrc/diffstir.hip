@@ -92,8 +92,8 @@ struct ds_ctx {
     // tiled kernels
     float4* d_vgrid = nullptr;     // grid velocity of the current step
     float4* d_pbar2 = nullptr;     // dL/d(momentum, mass) of the current adjoint step
-    float4* d_pfb = nullptr;       // fall-back momentum/mass grid (stencils outside their tile)
-    float4* d_vbfb = nullptr;      // fall-back dL/dv grid
+    double* d_pfb = nullptr;       // fall-back momentum/mass grid, FP64 x 4 per node (stencils outside their tile)
+    double* d_vbfb = nullptr;      // fall-back dL/dv grid, FP64 x 4 per node
     float4* d_staging = nullptr;   // per-chunk staging slots
     long staging_cap = 0;          // slots allocated
     float4* d_seg_grid = nullptr;
@@ -506,8 +506,8 @@ int alloc_mode(ds_ctx* c)
             return fail(c, DS_EINVAL, "too many particles for the tiled kernels in one context");
         MALLOC(c->d_vgrid, G);
         MALLOC(c->d_pbar2, G);
-        MALLOC(c->d_pfb, G);
-        MALLOC(c->d_vbfb, G);
+        MALLOC(c->d_pfb, 4 * G);
+        MALLOC(c->d_vbfb, 4 * G);
         MALLOC(c->d_seg_grid, (size_t)G * c->K);
         MALLOC(c->d_seg_aff, (size_t)9 * c->P * c->K);
         MALLOC(c->d_adj2, SF);
@@ -547,8 +547,8 @@ int alloc_mode(ds_ctx* c)
         c->staging_cap = std::max(c->maxch, c->maxgch);
         MALLOC(c->d_staging, (size_t)c->staging_cap * TNODES);
         fill_i32(c, c->d_fbflag, (long)c->B * c->NR * c->NR * c->NR, 0);
-        fill_zero_f4(c, c->d_pfb, G);
-        fill_zero_f4(c, c->d_vbfb, G);
+        fill_zero_f64(c, c->d_pfb, 4 * G);
+        fill_zero_f64(c, c->d_vbfb, 4 * G);
     }
 #undef MALLOC
     c->alloc_mode = c->mode;

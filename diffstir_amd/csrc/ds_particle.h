@@ -533,6 +533,75 @@ struct ScatterGlobal3 {
     }
 };
 
+// The fall-back scatter of the tiled kernels, with sums that do not depend on the order of the
+// atomic adds.  A particle whose contributions are below `bound` < 2^ex in magnitude adds each one
+// rounded to a multiple of its grain 2^(ex - 26) (an error below 2^-27 of the bound, under FP32's
+// own rounding) into an FP64 grid (4 doubles per node).  The rounding is (x + M) - M in FP64 with
+// M = 1.5 * 2^(ex + 26): x + M stays in M's binade, whose spacing is the grain.  Every partial sum
+// at a node is then a multiple of the smallest grain among the particles that reached it and
+// stays exact in FP64 (53-bit significands) while those particles' bounds lie within
+// 2^(26 - log2(count)) of each other, so the atomic adds give the same bits in any order.
+// Outside that range the sum is still exact to FP64 rounding.
+__device__ __forceinline__ double fallback_magic(float bound)
+{
+    int ex = 0;
+    (void)frexpf(bound, &ex);   // bound < 2^ex
+    return ldexp(1.5, ex + 26);
+}
+
+__device__ __forceinline__ double grain_round(float x, double M)
+{
+    return ((double)x + M) - M;
+}
+
+struct ScatterExact4 {
+    double* g;
+    int n;
+    double M;
+    __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c, float d)
+    {
+        double* q = g + 4 * node_index(n, i, j, k);
+        atomicAdd(q + 0, grain_round(a, M));
+        atomicAdd(q + 1, grain_round(b, M));
+        atomicAdd(q + 2, grain_round(c, M));
+        atomicAdd(q + 3, grain_round(d, M));
+    }
+};
+
+struct ScatterExact3 {
+    double* g;
+    int n;
+    double M;
+    __device__ __forceinline__ void add(int i, int j, int k, float a, float b, float c)
+    {
+        double* q = g + 4 * node_index(n, i, j, k);
+        atomicAdd(q + 0, grain_round(a, M));
+        atomicAdd(q + 1, grain_round(b, M));
+        atomicAdd(q + 2, grain_round(c, M));
+    }
+};
+
+// Bound on the P2G contributions of a particle with payload pl: W (mu + A d) per component with
+// W <= 1 and |d| <= 1.5 dx per axis, and W m.
+__device__ __forceinline__ float p2g_bound(const DevScene& sc, const P2GPayload& pl)
+{
+    float b = fabsf(pl.m);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        b = fmaxf(b, fabsf(pl.mu[c]) + 1.5f * sc.dx * (fabsf(pl.A[c * 3]) + fabsf(pl.A[c * 3 + 1]) + fabsf(pl.A[c * 3 + 2])));
+    return b * 1.001f;
+}
+
+// Bound on the adjoint G2P contributions W (vt + c4 Cb (o - fx)) with W <= 1 and |o - fx| <= 1.5.
+__device__ __forceinline__ float g2p_adj_bound(const DevScene& sc, const float vt[3], const float Cb[9])
+{
+    float b = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        b = fmaxf(b, fabsf(vt[a]) + 1.5f * fabsf(sc.c4) * (fabsf(Cb[a * 3]) + fabsf(Cb[a * 3 + 1]) + fabsf(Cb[a * 3 + 2])));
+    return b * 1.001f;
+}
+
 // velocity from a precomputed v_out grid
 struct GatherVout {
     const float4* g;

@@ -10,7 +10,8 @@
 // Each chunk works on an 8x8x8-node tile covering nodes [o-1, o+7) per axis (o = the block's
 // first cell): every stencil whose base cell lies in [o-1, o+4], so particles may drift one
 // cell out of their block within a segment.  A stencil that drifts further goes to a global
-// fall-back grid with atomics (rare).
+// fall-back grid with FP64 atomics of grain-rounded contributions (rare; exact sums, so their
+// order does not matter).
 //
 // Scatter without atomics.  LDS float atomics (ds_add_f32) run at about 1/24 of the rate of a
 // plain LDS read-modify-write on gfx950, and global float atomics are slow as well.  So:
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(WG, P2GMinWaves<CH>::value) k_p2g_tiled(const 
                                                   const int* __restrict__ rows,
                                                   const int* __restrict__ nchunks_p, int maxchunks,
                                                   const double* __restrict__ params, float4* __restrict__ staging,
-                                                  float4* __restrict__ fallback, int* __restrict__ fbflag, int NR,
+                                                  double* __restrict__ fallback, int* __restrict__ fbflag, int NR,
                                                   const double* __restrict__ jin, DevScene sc, int P,
                                                   unsigned* __restrict__ flag)
 {
@@ -337,7 +338,7 @@ __global__ void __launch_bounds__(WG, P2GMinWaves<CH>::value) k_p2g_tiled(const 
                 if (!it) {
                     mark_fallback_blocks(fbflag, ch.inst, s, NR);
                     if (sc.count_fallback) atomicAdd(reinterpret_cast<unsigned long long*>(flag + 2), 1ull);
-                    ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
+                    ScatterExact4 g{fallback + 4 * (long)ch.inst * nn, n, fallback_magic(p2g_bound(sc, pl))};
                     p2g_scatter(sc, s, pl, g);
                 }
             }
@@ -453,15 +454,19 @@ constexpr int REDUCE_BATCH = 16;
 // must call this; the result is meaningful in part 0 only.
 template <int NPART>
 __device__ __forceinline__ float4 staged_node_sum(const float4* __restrict__ staging, const int* s_nc, const int* s_cs,
-                                                  float4* __restrict__ fallback, bool fl, long gid, bool inb, int lx,
+                                                  double* __restrict__ fallback, bool fl, long gid, bool inb, int lx,
                                                   int ly, int lz, int ncomp)
 {
     __shared__ float4 s_part[NPART > 1 ? (NPART - 1) * 64 : 1];
     const int node = threadIdx.x & 63, part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (inb && fl && part == 0) {
-        acc = fallback[gid];
-        fallback[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // the fall-back grid's exact FP64 sums, rounded once
+        double2* f = reinterpret_cast<double2*>(fallback + 4 * gid);
+        const double2 a = f[0], b = f[1];
+        acc = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+        f[0] = make_double2(0.0, 0.0);
+        f[1] = make_double2(0.0, 0.0);
     }
     sum_covering_list<REDUCE_BATCH>(acc, staging, s_nc, s_cs, inb, lx, ly, lz, ncomp, part, NPART);
     if (NPART > 1) {
@@ -492,7 +497,7 @@ __device__ __forceinline__ float4 staged_node_sum(const float4* __restrict__ sta
 template <int NPART>
 __global__ void __launch_bounds__(64 * NPART) k_grid_reduce(const float4* __restrict__ staging,
                                                             const int* __restrict__ bnch, const int* __restrict__ cstart,
-                                                            const int* __restrict__ cov, float4* __restrict__ fallback,
+                                                            const int* __restrict__ cov, double* __restrict__ fallback,
                                                             int* __restrict__ fbflag, float4* __restrict__ pout,
                                                             float4* __restrict__ vout, DevScene sc, int NB,
                                                             int NR)
@@ -705,7 +710,7 @@ __global__ void __launch_bounds__(WG, 4) k_g2p_p2g_tiled(const float* in, const 
                                                          int maxchunks, const float4* __restrict__ vgrid,
                                                          const float* __restrict__ traj, int T, int f,
                                                          double* __restrict__ dvsum, const double* __restrict__ params,
-                                                         float4* __restrict__ staging, float4* __restrict__ fallback,
+                                                         float4* __restrict__ staging, double* __restrict__ fallback,
                                                          int* __restrict__ fbflag, int NR, DevScene sc, int P,
                                                          unsigned* __restrict__ flag)
 {
@@ -822,7 +827,7 @@ __global__ void __launch_bounds__(WG, 4) k_g2p_p2g_tiled(const float* in, const 
                 if (!it) {
                     mark_fallback_blocks(fbflag, ch.inst, s, NR);
                     if (sc.count_fallback) atomicAdd(reinterpret_cast<unsigned long long*>(flag + 2), 1ull);
-                    ScatterGlobal4 g{fallback + (long)ch.inst * nn, n};
+                    ScatterExact4 g{fallback + 4 * (long)ch.inst * nn, n, fallback_magic(p2g_bound(sc, pl))};
                     p2g_scatter(sc, s, pl, g);
                 }
             }
@@ -890,7 +895,7 @@ template <int PPT, class CH>
 __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ st, const float* __restrict__ adj,
                                                       const CH* __restrict__ chunks, const int* __restrict__ rows,
                                                       const int* __restrict__ nchunks_p, int maxchunks, const double* __restrict__ dvbar, int T, int f,
-                                                      float4* __restrict__ staging, float4* __restrict__ fallback,
+                                                      float4* __restrict__ staging, double* __restrict__ fallback,
                                                       int* __restrict__ fbflag, int NR, DevScene sc, int P)
 {
     // chunk count of the checkpoint, written by the bucket sort (bounded by the table size)
@@ -942,7 +947,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
                 for (int a = 0; a < 3; a++) vc[a] = vt[a];
                 if (!it) {
                     mark_fallback_blocks(fbflag, ch.inst, s, NR);
-                    ScatterGlobal3 g{fallback + (long)ch.inst * nn, n};
+                    ScatterExact3 g{fallback + 4 * (long)ch.inst * nn, n, fallback_magic(g2p_adj_bound(sc, vc, Cc))};
                     g2p_adj_scatter(sc, s, vc, Cc, g);
                 }
             }
@@ -975,7 +980,7 @@ __global__ void __launch_bounds__(WG) k_g2p_adj_tiled(const float* __restrict__ 
 template <int NPART>
 __global__ void __launch_bounds__(64 * NPART, 3) k_vbar_reduce(const float4* __restrict__ staging,
                                                             const int* __restrict__ bnch, const int* __restrict__ cstart,
-                                                            const int* __restrict__ cov, float4* __restrict__ fallback,
+                                                            const int* __restrict__ cov, double* __restrict__ fallback,
                                                             int* __restrict__ fbflag, const float4* __restrict__ grid,
                                                             float4* __restrict__ pbar, DevScene sc, int NB,
                                                             int NR)

@@ -232,9 +232,8 @@ def test_repeat_runs_are_bitwise_identical():
     contexts, replayed as hipGraphs, and with the adjoint's recompute on a second stream or on the
     main one.  The sort ranks are fixed by particle index, every grid node sums its staging slots
     in a fixed order, the stick-force partials are added as exact multiples of 2^-32 (order-free)
-    and the parameter-gradient partials are reduced in a fixed order.  The one order-dependent sum
-    left is the global fall-back scatter (FP32 atomics, for stencils outside their chunk's tile),
-    which this scene never takes (checked)."""
+    and the parameter-gradient partials are reduced in a fixed order.  This scene never takes the
+    fall-back scatter (checked; its own repeat test is below)."""
     sc = S.tiny_scene(16, 25, seed=6)
     out = []
     for graph, overlap in ((False, True), (False, True), (True, True), (False, False), (True, False)):
@@ -249,6 +248,35 @@ def test_repeat_runs_are_bitwise_identical():
             out.append((F.copy(), d.copy()))
     F0, d0 = out[0]
     assert np.abs(F0).max() > 0 and np.abs(d0).max() > 0
+    for F, d in out[1:]:
+        np.testing.assert_array_equal(F, F0)
+        np.testing.assert_array_equal(d, d0)
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_fallback_scatter_is_bitwise_repeatable(fused):
+    """With the tile window narrowed to 0 almost every stencil takes the fall-back scatter, and
+    each fall-back node sums contributions of many particles with global atomics in whatever
+    order the hardware runs them.  Those contributions are rounded to a per-particle grain and
+    summed in FP64, where the sums are exact: forces and parameter gradient have the same bits in
+    fresh contexts, with graph replay and with the adjoint's recompute on one stream."""
+    sc, z = gold_scene("plain_n32_20")
+    n = int(z["nsteps"])
+    out = []
+    for graph, overlap in ((False, True), (False, True), (True, True), (False, False)):
+        print(f"[gpu] fall-back bitwise repeat, fused {fused}, graph {graph}, overlap {overlap}", flush=True)
+        with Engine([sc], ckpt_every=7, max_steps=n + 1, graph=graph, overlap=overlap) as e:
+            e.configure(fused_step=fused)
+            e.set_diagnostics(tile_window=0, count_fallback=True)
+            e.reset()
+            e.forward()
+            F = e.forces()
+            d = e.backward(np.ones_like(F))
+            assert e.fallback_count() > 0.5 * n * (sc.n_liquid + sc.n_stick + sc.n_container)
+            out.append((F.copy(), d.copy()))
+    F0, d0 = out[0]
+    assert np.abs(F0).max() > 0 and np.abs(d0).max() > 0
+    assert force_err(F0[0], z["forces"]) <= FORCE_TOL
     for F, d in out[1:]:
         np.testing.assert_array_equal(F, F0)
         np.testing.assert_array_equal(d, d0)
