@@ -1014,7 +1014,7 @@ __global__ void __launch_bounds__(64 * NPART, 3) k_vbar_reduce(const float4* __r
 // written in place of the next step's, and the parameter gradient added to tb.
 __device__ __forceinline__ void p2g_adj_stress_at(const DevScene& sc, const InstParams& ip,
                                                   const float* __restrict__ st, float* __restrict__ adj,
-                                                  const double* __restrict__ mj, long LP, int p, const float xp[3],
+                                                  const double M[9], double J, long LP, int p, const float xp[3],
                                                   const float ub[3], const float Ab[9], const float fxb[3],
                                                   double tb[4])
 {
@@ -1027,11 +1027,8 @@ __device__ __forceinline__ void p2g_adj_stress_at(const DevScene& sc, const Inst
     float Fnb[9], xo[3], vo[3], Co[9], Fo[9];
 #pragma unroll
     for (int a = 0; a < 9; a++) Fnb[a] = adj[(SF + a) * LP + p];
-    double M[9];
-#pragma unroll
-    for (int a = 0; a < 9; a++) M[a] = mj[a * LP + p];
     LiquidP2G L;
-    liquid_stress(sc, ip.th, ip.kb, q.C, mj[9 * LP + p], L);
+    liquid_stress(sc, ip.th, ip.kb, q.C, J, L);
     p2g_adj_stress(sc, ip, q, L, M, ub, Ab, fxb, Fnb, xp, xo, vo, Co, Fo, tb);
 #pragma unroll
     for (int a = 0; a < 3; a++) adj[(SX + a) * LP + p] = xo[a];
@@ -1114,6 +1111,25 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
             stencil(sc, x[0], x[1], x[2], s);
             const bool it = stencil_in_tile(s, x0, y0, z0, sc.tile_win);
             if (ch.species == LIQUID) {
+                // the affine stored by the recompute pass and the velocity, loaded before the
+                // position-gradient gather so that their latency overlaps it
+                float A[9];
+#pragma unroll
+                for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
+                const float u[3] = {st[SV * LP + p], st[(SV + 1) * LP + p] + sc.gdt, st[(SV + 2) * LP + p]};
+                // with the position gradient computed here as well and two particles per thread
+                // or more (two wavefronts per SIMD either way) the SVD backward factor and J are
+                // loaded this early too; otherwise they are loaded where they are used, which keeps
+                // the other forms at their occupancy (three wavefronts per SIMD with the gradient on
+                // its own stream, two with one particle per thread)
+                constexpr bool kEarlyMJ = kFused && !kXp && PPT > 1;
+                double M[9], J = 1.0;
+                auto load_mj = [&]() {
+#pragma unroll
+                    for (int a = 0; a < 9; a++) M[a] = mj[a * LP + p];
+                    J = mj[9 * LP + p];
+                };
+                if (kEarlyMJ) load_mj();
                 float xp[3];
                 if (kXp) {
 #pragma unroll
@@ -1148,11 +1164,8 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                     }
                 }
                 asm volatile("" ::: "memory");
-                // the gather with the affine stored by the recompute pass
-                float A[9], ub[3], Ab[9], fxb[3];
-#pragma unroll
-                for (int a = 0; a < 9; a++) A[a] = ain[a * LP + p];
-                const float u[3] = {st[SV * LP + p], st[(SV + 1) * LP + p] + sc.gdt, st[(SV + 2) * LP + p]};
+                // the gather with the affine
+                float ub[3], Ab[9], fxb[3];
                 if (it) {
                     TileQ4 g{tq, x0, y0, z0};
                     p2g_adj_gather(sc, s, A, u, g, ub, Ab, fxb);
@@ -1162,7 +1175,8 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_tiled(const float* __restrict__ 
                 }
                 if (kFused) {
                     asm volatile("" ::: "memory");
-                    p2g_adj_stress_at(sc, load_params(sc, params, ch.inst), st, adj, mj, LP, p, xp, ub, Ab, fxb,
+                    if (!kEarlyMJ) load_mj();
+                    p2g_adj_stress_at(sc, load_params(sc, params, ch.inst), st, adj, M, J, LP, p, xp, ub, Ab, fxb,
                                       tb);
                 } else {
 #pragma unroll
@@ -1277,7 +1291,10 @@ __global__ void __launch_bounds__(WG) k_p2g_adj_stress(const float* __restrict__
             }
 #pragma unroll
             for (int a = 0; a < 9; a++) Ab[a] = gin[(6 + a) * LP + p];
-            p2g_adj_stress_at(sc, ip, st, adj, mj, LP, p, xp, ub, Ab, fxb, tb);
+            double M[9];
+#pragma unroll
+            for (int a = 0; a < 9; a++) M[a] = mj[a * LP + p];
+            p2g_adj_stress_at(sc, ip, st, adj, M, mj[9 * LP + p], LP, p, xp, ub, Ab, fxb, tb);
         }
         add_chunk_partials(red, tb, thpart, c);
         __syncthreads();
