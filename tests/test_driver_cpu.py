@@ -217,3 +217,48 @@ def test_levenberg_marquardt_bookkeeping(monkeypatch, fail_on):
     monkeypatch.setattr(_ModelEngine, "fail_on", (1,))
     with pytest.raises(DiffStirError):
         I.identify_lm(None, _ModelEngine.profile(truth), inits, iterations=2)
+
+
+def test_sensitivity_on_a_closed_form_model(monkeypatch):
+    """identify.sensitivity on the closed-form force model: the Jacobian of the x/z entries matches
+    the analytic derivative in log-parameters, the Gauss-Newton matrix is J^T J / (T-1) with a
+    positive spectrum, the predicted standard deviations follow from the noise level, and the
+    loss scan and point losses are the force MSE against the profile at theta."""
+    from diffstir_amd import identify as I
+    monkeypatch.setattr(_ModelEngine, "calls", 0)
+    monkeypatch.setattr(_ModelEngine, "fail_on", ())
+    monkeypatch.setattr(I, "Engine", _ModelEngine)
+    theta = np.array([2.0, 15.0, 1.4, 1000.0])
+    T = _ModelEngine.T
+    res = I.sensitivity(None, theta, rel_step=1e-4, noise=0.01, scan=[-0.2, 0.0, 0.2],
+                        points=[theta[:3] * 1.1])
+    t = np.arange(T) / T
+    K, tau, n = theta[:3]
+    # analytic d(entries)/d(log theta) of the model's x (rows 0..T-2) and z entries
+    dx = np.stack([K * np.exp(-n * t), tau * t ** 2, -n * t * K * np.exp(-n * t)], axis=1)[:T - 1]
+    dz = np.stack([K * tau * np.sin(5 * t), K * tau * np.sin(5 * t), np.full(T, n)], axis=1)[:T - 1]
+    J = np.concatenate([dx, dz])
+    np.testing.assert_allclose(res["jacobian"], J, rtol=1e-6, atol=1e-6)
+    G = J.T @ J / (T - 1)
+    np.testing.assert_allclose(res["gauss_newton"], G, rtol=1e-6)
+    w, V = np.linalg.eigh(G)
+    np.testing.assert_allclose(res["eigvals"], w, rtol=1e-6)
+    F0 = _ModelEngine.profile(theta)
+    sigma = 0.01 * np.sqrt(np.mean(F0[2:] ** 2))
+    assert res["noise_sigma"] == pytest.approx(sigma)
+    np.testing.assert_allclose(res["std_log"], np.sqrt(np.diag(sigma ** 2 * np.linalg.inv(J.T @ J))), rtol=1e-5)
+
+    def loss_at(p):
+        F = _ModelEngine.profile(p)
+        d = np.concatenate([F[:T - 1, 0] - F0[:T - 1, 0], F[:T - 1, 2] - F0[:T - 1, 2]])
+        return np.sum(d * d) / (2 * (T - 1))
+
+    weak = np.asarray(res["eigvecs"])[:, 0]
+    for tt, L in zip(res["scan_t"], res["scan_loss"]):
+        p = theta.copy()
+        p[:3] *= np.exp(tt * weak)
+        assert L == pytest.approx(loss_at(p), rel=1e-9, abs=1e-15)
+    p = theta.copy()
+    p[:3] *= 1.1
+    assert res["point_loss"][0] == pytest.approx(loss_at(p), rel=1e-9)
+    assert res["point_chi2"][0] == pytest.approx(2 * (T - 1) * loss_at(p) / sigma ** 2, rel=1e-9)
