@@ -212,7 +212,9 @@ def test_batched_identification_reduces_the_loss():
 
 def test_levenberg_marquardt_recovers_noiseless_parameters():
     """identify_lm from two starts 20-30 % off the ground truth, against the noiseless profile:
-    the best loss never rises and falls by at least 1e4 within 8 iterations."""
+    the best loss never rises and falls by at least 1e4 within 8 iterations, to parameters within
+    2 % of the truth (measured on MI355X: 0.6-0.8 % in the worst-determined parameter, where the
+    FP32 forces' rounding leaves the loss flat)."""
     from diffstir_amd.identify import identify_lm, synthetic_target
     truth = S.hb_params_draw(np.random.default_rng(5))
     base = S.tiny_scene(16, 20, seed=0, params=truth)
@@ -226,6 +228,7 @@ def test_levenberg_marquardt_recovers_noiseless_parameters():
     assert L.shape == (9, 2)
     assert np.all(np.diff(L, axis=0) <= 0)
     assert np.all(L[-1] <= 1e-4 * L[0])
+    assert np.all(np.abs(res["final"][:, :3] - truth[:3]) <= 0.02 * truth[:3])
 
 
 def test_sensitivity_jacobian_agrees_with_the_adjoint():
