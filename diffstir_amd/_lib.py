@@ -24,6 +24,7 @@ KERNEL_NAMES = ("k_p2g_tiled", "k_grid_reduce", "k_g2p_tiled", "k_g2p_adj_tiled"
                 "k_g2p_p2g_tiled")
 
 ERROR_NAMES = {0: "DS_OK", 1: "DS_EINVAL", 2: "DS_EHIP", 3: "DS_ENOMEM", 4: "DS_ESTATE", 5: "DS_ENONFINITE"}
+DS_ENONFINITE = 5
 
 
 class LibraryNotBuilt(RuntimeError):

@@ -128,8 +128,7 @@ def identify_lm(scene: Scene, target: np.ndarray, init_params: np.ndarray, itera
         nxt[idx] = th[idx] * np.exp(d)
         trial[b] = nxt
 
-    from ._lib import DiffStirError
-    nonfinite = 5  # DS_ENONFINITE
+    from ._lib import DS_ENONFINITE, DiffStirError
     with Engine([scene] * (R * per), ckpt_every=ckpt_every) as e:
         for it in range(iterations + 1):
             pts = []
@@ -146,7 +145,7 @@ def identify_lm(scene: Scene, target: np.ndarray, init_params: np.ndarray, itera
                 e.forward()
                 F = e.forces()
             except DiffStirError as err:
-                if err.code != nonfinite or any(bb is None for bb in best):
+                if err.code != DS_ENONFINITE or any(bb is None for bb in best):
                     raise
                 F = None
             for b in range(R):
