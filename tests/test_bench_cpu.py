@@ -43,3 +43,30 @@ def test_trace_summary_samples_the_first_dispatch_after_each_hold(tmp_path, caps
     assert b["dispatches"] == 2 and abs(b["avg_us"] - 25.0) < 1e-9
     assert b["sampled"] == 1 and abs(b["sampled_avg_us"] - 20.0) < 1e-9
     assert "k_hold" not in out
+
+
+def test_roofline_of_the_dominant_sampled_kernel():
+    """The dominant kernel is the one with the most sampled time; its launch time is the event
+    sample minus the dispatch gap; achieved = algorithmic bytes per launch / launch time; the
+    traffic comes from the committed counter summary only when kernel and options match."""
+    with open(os.path.join(ROOT, "profiles", "traffic_single_n64.json")) as f:
+        t = json.load(f)
+    opts = dict(t["options"])
+    n_all = 149932
+    r = {"workload": "single_n64", "options": opts, "n_all": n_all,
+         "kernels": {"k_g2p_p2g_tiled": (200 * 34.7e-3, 200), "k_p2g_tiled": (240 * 27.0e-3, 240),
+                     "k_g2p_tiled": (0.0, 0), "bucket_sort": (10 * 90e-3, 10)}}
+    out = bench.roofline(r)
+    assert out["kernel"] == "k_g2p_p2g_tiled"
+    assert abs(out["event_sample_us"] - 34.7) < 1e-9
+    assert abs(out["avg_launch_us"] - (34.7 - bench.DISPATCH_GAP_US)) < 1e-9
+    per_launch = bench.KERNEL_BYTES_PER_PARTICLE["k_g2p_p2g_tiled"] * n_all
+    assert out["algorithmic_bytes_per_launch"] == per_launch
+    gbs = per_launch / ((34.7 - bench.DISPATCH_GAP_US) * 1e-6) / 1e9
+    assert abs(out["achieved"] - gbs) < 1e-6 and abs(out["frac"] - gbs / 8000.0) < 1e-12
+    assert out["peak"] == 8000.0 and out["unit"] == "GB/s" and out["bound"] == "hbm"
+    assert out["traffic"] == t["bytes_per_launch"]
+    assert out["kernel_avg_us"]["k_g2p_tiled"] is None
+    # other options: the committed counters do not describe this run
+    r2 = dict(r, options=dict(opts, particles_per_thread=4))
+    assert bench.roofline(r2)["traffic"] is None
