@@ -535,8 +535,8 @@ struct ScatterGlobal3 {
 
 // The fall-back scatter of the tiled kernels, with sums that do not depend on the order of the
 // atomic adds.  A particle whose contributions are below `bound` < 2^ex in magnitude adds each one
-// rounded to a multiple of its grain 2^(ex - 26) (an error below 2^-27 of the bound, under FP32's
-// own rounding) into an FP64 grid (4 doubles per node).  The rounding is (x + M) - M in FP64 with
+// rounded to a multiple of its grain 2^(ex - 26) (an error of at most half the grain, under 2^-26
+// of the bound and so under FP32's own rounding) into an FP64 grid (4 doubles per node).  The rounding is (x + M) - M in FP64 with
 // M = 1.5 * 2^(ex + 26): x + M stays in M's binade, whose spacing is the grain.  Every partial sum
 // at a node is then a multiple of the smallest grain among the particles that reached it and
 // stays exact in FP64 (53-bit significands) while those particles' bounds lie within
