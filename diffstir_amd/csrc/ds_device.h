@@ -296,22 +296,67 @@ __device__ __forceinline__ void svd3(const double F[9], double U[9], double S[3]
     }
 }
 
-// The singular values alone, in the same signed convention as svd3: S = sqrt of the
-// eigenvalues of F^T F from the same Jacobi sweeps (no V accumulated), sorted descending, with
-// the sign of det F on the smallest.  S[0]*S[1]*S[2] = det F up to rounding.
+// The singular values alone, in the same signed convention as svd3: S[0], S[1] = sqrt of the two
+// largest eigenvalues of A = F^T F, S[2] = det F / (S[0] S[1]) (so the smallest carries the sign
+// of det F, and S[0]*S[1]*S[2] = det F up to rounding).  The eigenvalues come in closed form (O. K. Smith,
+// "Eigenvalues of a symmetric 3 x 3 matrix", Comm. ACM 4(4), 1961): with q = tr(A)/3,
+// B = A - q I, p = sqrt(tr(B^2)/6) and r = det(B)/(2 p^3) in [-1, 1], they are
+// q + 2 p cos(phi + 2 pi k / 3), k = 0, 1, 2, phi = acos(r)/3 (cos(phi) from the triple-angle
+// cubic below, without trigonometric calls).  B is formed exactly enough
+// for F near a rotation (a_ii - q loses nothing), and the sum of the three is 3q, so the
+// middle one is taken from the other two.  Errors are of order 1e-16 of the largest
+// eigenvalue; where two eigenvalues nearly coincide their individual errors grow (to about
+// 1e-12 of the largest for a coincident pair at the top, more for a coincident pair below a
+// much larger one), which leaves the product with S[2] below unaffected.
 __device__ __forceinline__ void svd3_values(const double F[9], double S[3])
 {
-    double A[3][3], V[3][3];
-    jacobi_eig_FtF<false>(F, A, V);
-    const double l0 = fmax(A[0][0], 0.0), l1 = fmax(A[1][1], 0.0), l2 = fmax(A[2][2], 0.0);
+    double A[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i; j < 3; j++)
+            A[i][j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
+    const double a01 = A[0][1], a02 = A[0][2], a12 = A[1][2];
+    const double q = (A[0][0] + A[1][1] + A[2][2]) * (1.0 / 3.0);
+    const double b00 = A[0][0] - q, b11 = A[1][1] - q, b22 = A[2][2] - q;
+    const double off = a01 * a01 + a02 * a02 + a12 * a12;
+    const double p2 = (b00 * b00 + b11 * b11 + b22 * b22 + 2.0 * off) * (1.0 / 6.0);
+    double l0 = q, l1 = q, l2 = q;
+    if (p2 > 0.0) {
+        const double p = sqrt(p2);
+        const double detB = b00 * (b11 * b22 - a12 * a12) - a01 * (a01 * b22 - a12 * a02) + a02 * (a01 * a12 - b11 * a02);
+        const double r = fmin(1.0, fmax(-1.0, detB * rcp64(2.0 * p2 * p)));
+        // c = cos(acos(r) / 3), the root in [1/2, 1] of 4 c^3 - 3 c = r, by Newton's method from
+        // above (the cubic is convex and increasing there, so the iterates fall monotonically);
+        // the start 1/2 + sqrt((1 + r) / 6) is the root's expansion at r = -1 and lies above it
+        double c = 0.5 + sqrt((1.0 + r) * (1.0 / 6.0));
+#pragma unroll
+        for (int it = 0; it < 5; it++) {
+            const double g = (4.0 * c * c - 3.0) * c - r;
+            const double gp = 12.0 * c * c - 3.0;
+            c -= gp > 0.0 ? g * rcp64(gp) : 0.0;
+        }
+        // cos(phi + 2 pi / 3) = -c / 2 - (sqrt(3) / 2) sin(phi), sin(phi) = sqrt(1 - c^2)
+        const double sn = sqrt(fmax(0.0, 1.0 - c * c));
+        l0 = q + 2.0 * p * c;
+        l2 = q - p * (c + 1.7320508075688772 * sn);
+        l1 = 3.0 * q - l0 - l2;
+    }
+    l0 = fmax(l0, 0.0);
+    l1 = fmax(l1, 0.0);
+    l2 = fmax(l2, 0.0);
     const double hi = fmax(fmax(l0, l1), l2);
-    const double lo = fmin(fmin(l0, l1), l2);
     const double mid = fmax(fmin(l0, l1), fmin(fmax(l0, l1), l2));
     const double detF = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6])
                       + F[2] * (F[3] * F[7] - F[4] * F[6]);
     S[0] = hi > 0.0 ? hi * rsqrt64(hi) : 0.0;
     S[1] = mid > 0.0 ? mid * rsqrt64(mid) : 0.0;
-    const double s2 = lo > 0.0 ? lo * rsqrt64(lo) : 0.0;
+    // the smallest from |det F| = S0 S1 S2: the closed form's eigenvalues carry errors of order
+    // 1e-16 of the largest, which the smallest cannot afford when F is far from a rotation (a
+    // liquid sheared by a long rollout: S0 / S2 of 1e4 and more); this keeps S2, and the product
+    // J that the stress uses, accurate to rounding
+    const double s01 = S[0] * S[1];
+    const double s2 = s01 > 0.0 ? fabs(detF) / s01 : 0.0;
     S[2] = detF < 0.0 ? -s2 : s2;
 }
 
