@@ -354,9 +354,11 @@ __device__ __forceinline__ void svd3_values(const double F[9], double S[3])
     // the smallest from |det F| = S0 S1 S2: the closed form's eigenvalues carry errors of order
     // 1e-16 of the largest, which the smallest cannot afford when F is far from a rotation (a
     // liquid sheared by a long rollout: S0 / S2 of 1e4 and more); this keeps S2, and the product
-    // J that the stress uses, accurate to rounding
+    // J that the stress uses, accurate to rounding; capped at S1, which it can only pass when F
+    // is numerically rank-deficient (S1 then holds rounding noise and |det F| / S0 S1 is noise
+    // divided by noise)
     const double s01 = S[0] * S[1];
-    const double s2 = s01 > 0.0 ? fabs(detF) / s01 : 0.0;
+    const double s2 = s01 > 0.0 ? fmin(fabs(detF) / s01, S[1]) : 0.0;
     S[2] = detF < 0.0 ? -s2 : s2;
 }
 

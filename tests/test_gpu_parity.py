@@ -394,6 +394,53 @@ def test_svd3_values_match_lapack_and_the_full_svd():
     assert np.all(np.sign(S[:, 2]) * np.sign(np.linalg.det(F)) >= 0)
 
 
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def test_svd3_values_on_hard_spectra():
+    """Spectra where a closed form is most fragile, with known singular values: an equal leading
+    pair (the triple-angle cubic's double root at cos = 1/2), an equal trailing pair (its simple
+    root at 1), three equal values, rank one and two, six decades of range, and plain shears
+    (whose smallest value comes from the determinant). Each F = Q1 diag(s) Q2^T with random
+    rotations, one column sign flipped for half of them. The forward substep uses only their
+    product J = S0 S1 S2, which comes from det F and must hold to rounding: 1e-15 s0^3. Each value
+    is held to 3e-8 s0. The closed form is less accurate than LAPACK for single values here
+    (measured on gfx950: about 2e-9 s0 at an equal leading pair, where the cubic has a double root,
+    and 1.1e-8 s0 at a condition number of 1e6), but a dropped term, a wrong root of the cubic or a
+    swapped pair misses either bound by orders of magnitude. """
+    rng = np.random.default_rng(5)
+    spectra = [(2.0, 2.0, 0.5), (1.0, 1.0, 1e-6), (3.0, 1e-4, 1e-4), (1.5, 0.7, 0.7),
+               (1.0, 1.0, 1.0), (1.0, 0.0, 0.0), (2.0, 1.0, 0.0), (1e3, 1.0, 1e-3),
+               (1.0, 1.0 - 1e-9, 1.0 - 2e-9), (1.0 + 1e-12, 1.0, 1e-12)]
+    F, S_true = [], []
+    for s in spectra:
+        for k in range(8):
+            d = np.diag(s)
+            if k % 2:
+                d = d @ np.diag([1.0, 1.0, -1.0])
+            F.append(_rotation(rng) @ d @ _rotation(rng).T)
+            S_true.append(s)
+    for k in (0.0, 0.3, 1.0, 5.0, 40.0):        # shears, unit determinant
+        F.append(np.array([[1.0, k, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]))
+        S_true.append(np.linalg.svd(F[-1], compute_uv=False))
+    F = np.ascontiguousarray(np.array(F))
+    S_true = np.array(S_true, dtype=np.float64)
+    S = svd3_values(F)
+    s0 = S_true[:, 0]
+    det = np.linalg.det(F)
+    assert np.all(np.abs(S.prod(1) - det) <= 1e-15 * s0 ** 3)
+    tol = 3e-8 * s0[:, None]
+    assert np.all(np.abs(np.abs(S) - S_true) <= tol)
+    assert np.all(S[:, 0] >= S[:, 1]) and np.all(S[:, 1] >= np.abs(S[:, 2]))
+    clear = np.abs(det) > 1e-10 * s0 ** 3
+    assert np.all(np.sign(S[clear, 2]) == np.sign(det[clear]))
+    _, Sf, _ = svd3(F)
+    assert np.all(np.abs(S - Sf) <= 2 * tol)
+
+
 def test_svd3_backward_matches_torch_autograd():
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(1)
